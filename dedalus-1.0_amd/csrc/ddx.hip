@@ -1,0 +1,877 @@
+// libddx: C ABI of the MI355X pseudospectral step (declarations in include/ddx.h).
+//
+// Host orchestration of the kernels in ddx_kernels.h.  One plan per grid;
+// every launch goes to the plan's stream; argument and shape checks happen on
+// the host before any launch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ddx.h"
+#include "ddx_kernels.h"
+
+using namespace ddx;
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = -1;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(DDX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define LAUNCHCHK()                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = hipGetLastError();                                                     \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(DDX_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_));    \
+  } while (0)
+
+// Per-axis wavenumbers, computed exactly as the reference's numpy does
+// (representations.py:204-226): k = fftfreq(n)[m] * 2 * kny with
+// kny = pi n / L and the Nyquist entry (even n) sign-flipped to +kny; the
+// 2/3 rule keeps |k| < (2/3) kny (representations.py:384-397).
+struct AxisInfo {
+  int n = 1;
+  double L = 1.0;
+  int c = 0;  // largest kept |frequency index|
+  int K = 1;  // kept count: 2c+1 on full axes, c+1 on the x axis
+  std::vector<double> kfull;  // k for frequency index m in [0, n)
+};
+
+int setup_axis(int n, double L, bool half, AxisInfo* ax, std::string* why) {
+  ax->n = n;
+  ax->L = L;
+  const double kny = (M_PI * (double)n) / L;
+  const double inv = 1.0 / (double)n;
+  const double thr = 2.0 / 3.0 * kny;
+  ax->kfull.resize(n);
+  const int npos = (n - 1) / 2 + 1;  // numpy fftfreq split
+  std::vector<int> keep(n, 0);
+  for (int m = 0; m < n; ++m) {
+    const int ms = (m < npos) ? m : m - n;
+    double k = ((double)ms * inv) * 2.0 * kny;
+    if (n % 2 == 0 && m == n / 2) k = -k;
+    ax->kfull[m] = k;
+    keep[m] = std::fabs(k) < thr;
+  }
+  int c = -1;
+  while (c + 1 <= n / 2 && keep[c + 1]) ++c;
+  if (c < 0) {
+    *why = "2/3 rule removes the mean mode";
+    return -1;
+  }
+  // check the kept set is {0..c} (x axis: within the half spectrum) or {0..c} U {n-c..n-1}
+  const int hmax = half ? n / 2 : n - 1;
+  for (int m = 0; m <= hmax; ++m) {
+    bool exp_keep = (m <= c) || (!half && m >= n - c);
+    if (!!keep[m] != exp_keep) {
+      *why = "2/3 kept set is not symmetric";
+      return -1;
+    }
+  }
+  ax->c = c;
+  ax->K = half ? c + 1 : 2 * c + 1;
+  return 0;
+}
+
+}  // namespace
+
+struct ddx_plan {
+  int device = 0;
+  int ndim = 2;
+  int precision = DDX_F64;
+  AxisInfo ax, ay, az;  // az: n = 1 in 2D
+  int Kx = 1, Kxp = 1, Ky = 1, Kz = 1;
+  long nk = 0;          // Ky*Kz*Kxp
+  long npts = 0;        // nx*ny*nz
+  long nref = 0;        // reference k elements
+  FftPlan fx, fy, fz;
+  double* dkx = nullptr;
+  double* dky = nullptr;
+  double* dkz = nullptr;
+  std::map<int, void*> tw;  // n -> twiddle table (precision of the plan)
+  std::vector<void*> owned;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  void* wsA = nullptr;
+  void* wsB = nullptr;
+  size_t ws_bytes = 0;
+};
+
+namespace {
+
+size_t cpx_bytes(const ddx_plan* p) { return p->precision == DDX_F64 ? 16 : 8; }
+size_t real_bytes(const ddx_plan* p) { return p->precision == DDX_F64 ? 8 : 4; }
+
+KGeom kgeom(const ddx_plan* p) {
+  KGeom g;
+  g.Ky = p->Ky;
+  g.Kz = p->Kz;
+  g.Kx = p->Kx;
+  g.Kxp = p->Kxp;
+  g.kx = p->dkx;
+  g.ky = p->dky;
+  g.kz = p->dkz;
+  return g;
+}
+
+int ensure_ws(ddx_plan* p, int nfields) {
+  // both buffers hold nfields arrays of the largest mixed layout
+  const long ny = p->ay.n, nz = p->az.n;
+  long per = std::max((long)p->Ky * nz, nz * ny) * p->Kxp;
+  per = std::max(per, (long)p->nk);
+  const size_t need = (size_t)nfields * per * cpx_bytes(p);
+  if (need <= p->ws_bytes) return 0;
+  if (p->wsA) HIPCHK(hipFree(p->wsA));
+  if (p->wsB) HIPCHK(hipFree(p->wsB));
+  p->wsA = p->wsB = nullptr;
+  p->ws_bytes = 0;
+  if (hipMalloc(&p->wsA, need) != hipSuccess || hipMalloc(&p->wsB, need) != hipSuccess)
+    return fail(DDX_ERR_NOMEM, "workspace allocation failed");
+  p->ws_bytes = need;
+  return 0;
+}
+
+long ws_field_stride(const ddx_plan* p) {
+  const long ny = p->ay.n, nz = p->az.n;
+  long per = std::max((long)p->Ky * nz, nz * ny) * p->Kxp;
+  return std::max(per, (long)p->nk);
+}
+
+int tile_width(int n, size_t cb) {
+  // two ping-pong tiles of n x W complex within 64 KiB of LDS
+  int W = 8;
+  while (W > 1 && (size_t)2 * n * W * cb > 65536) W /= 2;
+  return W;
+}
+
+// ---------------------------------------------------------------------------
+// column passes
+// ---------------------------------------------------------------------------
+
+enum ColKind { COL_PLAIN, COL_SPECTRAL };
+
+template <typename T>
+int launch_col(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
+  if (nfields < 1 || nfields > DDX_MAXF) return fail(DDX_ERR_ARG, "bad field count");
+  a.W = tile_width(a.n, sizeof(cpx<T>));
+  const size_t lds = (size_t)2 * a.n * a.W * sizeof(cpx<T>);
+  if (lds > 65536) return fail(DDX_ERR_UNSUPPORTED, "column FFT too long for LDS");
+  dim3 grid((a.ncols + a.W - 1) / a.W, nslabs, nfields);
+  dim3 block(256);
+  if (dir > 0) {
+    if (spectral) hipLaunchKernelGGL((col_fft_kernel<T, 1, true>), grid, block, lds, p->stream, a);
+    else hipLaunchKernelGGL((col_fft_kernel<T, 1, false>), grid, block, lds, p->stream, a);
+  } else {
+    if (spectral) hipLaunchKernelGGL((col_fft_kernel<T, -1, true>), grid, block, lds, p->stream, a);
+    else hipLaunchKernelGGL((col_fft_kernel<T, -1, false>), grid, block, lds, p->stream, a);
+  }
+  LAUNCHCHK();
+  return 0;
+}
+
+template <typename T>
+void col_defaults(ColArgs<T>& a) {
+  memset(&a, 0, sizeof(a));
+  a.in_c = -1;
+  a.out_c = -1;
+  a.row_dim = -1;
+  a.slab_dim = -1;
+  a.scale = T(1);
+}
+
+// Backward z pass (3D): compact (Ky, Kz, Kxp) -> (Ky, Nz, Kxp)
+template <typename T>
+void geom_bz(const ddx_plan* p, ColArgs<T>& a) {
+  a.fp = p->fz;
+  a.tw = (const cpx<T>*)p->tw.at(p->az.n);
+  a.n = p->az.n;
+  a.in_rows = p->Kz; a.in_c = p->az.c;
+  a.out_rows = p->az.n; a.out_c = -1;
+  a.ncols = p->Kxp;
+  a.in_rs = p->Kxp; a.in_ss = (long)p->Kz * p->Kxp;
+  a.out_rs = p->Kxp; a.out_ss = (long)p->az.n * p->Kxp;
+  a.k_row = p->dkz; a.row_dim = 2;
+  a.k_slab = p->dky; a.slab_dim = 1;
+  a.k_col = p->dkx;
+}
+
+// Backward y pass.  3D: (Ky, Nz, Kxp) -> (Nz, Ny, Kxp); 2D: (Ky, Kxp) -> (Ny, Kxp)
+template <typename T>
+void geom_by(const ddx_plan* p, ColArgs<T>& a) {
+  const long nz = p->az.n;
+  a.fp = p->fy;
+  a.tw = (const cpx<T>*)p->tw.at(p->ay.n);
+  a.n = p->ay.n;
+  a.in_rows = p->Ky; a.in_c = p->ay.c;
+  a.out_rows = p->ay.n; a.out_c = -1;
+  a.ncols = p->Kxp;
+  a.in_rs = nz * p->Kxp; a.in_ss = p->Kxp;
+  a.out_rs = p->Kxp; a.out_ss = (long)p->ay.n * p->Kxp;
+  a.k_row = p->dky; a.row_dim = 1;
+  a.k_slab = nullptr; a.slab_dim = -1;
+  a.k_col = p->dkx;
+}
+
+// Forward y pass.  3D: (Nz, Ny, Kxp) -> (Ky, Nz, Kxp); 2D: (Ny, Kxp) -> (Ky, Kxp)
+template <typename T>
+void geom_fy(const ddx_plan* p, ColArgs<T>& a) {
+  const long nz = p->az.n;
+  a.fp = p->fy;
+  a.tw = (const cpx<T>*)p->tw.at(p->ay.n);
+  a.n = p->ay.n;
+  a.in_rows = p->ay.n; a.in_c = -1;
+  a.out_rows = p->Ky; a.out_c = p->ay.c;
+  a.ncols = p->Kxp;
+  a.in_rs = p->Kxp; a.in_ss = (long)p->ay.n * p->Kxp;
+  a.out_rs = nz * p->Kxp; a.out_ss = p->Kxp;
+}
+
+// Forward z pass (3D): (Ky, Nz, Kxp) -> (Ky, Kz, Kxp)
+template <typename T>
+void geom_fz(const ddx_plan* p, ColArgs<T>& a) {
+  a.fp = p->fz;
+  a.tw = (const cpx<T>*)p->tw.at(p->az.n);
+  a.n = p->az.n;
+  a.in_rows = p->az.n; a.in_c = -1;
+  a.out_rows = p->Kz; a.out_c = p->az.c;
+  a.ncols = p->Kxp;
+  a.in_rs = p->Kxp; a.in_ss = (long)p->az.n * p->Kxp;
+  a.out_rs = p->Kxp; a.out_ss = (long)p->Kz * p->Kxp;
+}
+
+template <typename T>
+int launch_row(ddx_plan* p, RowArgs<T>& a) {
+  const int nseq = std::max((a.nin + 1) / 2, (a.nout + 1) / 2);
+  const size_t lds = (size_t)2 * nseq * a.nx * sizeof(cpx<T>);
+  if (lds > 65536) return fail(DDX_ERR_UNSUPPORTED, "x rows too long for LDS");
+  const long nrows = (long)p->ay.n * p->az.n;
+  hipLaunchKernelGGL((row_kernel<T>), dim3(nrows), dim3(256), lds, p->stream, a);
+  LAUNCHCHK();
+  return 0;
+}
+
+template <typename T>
+void row_defaults(const ddx_plan* p, RowArgs<T>& a) {
+  memset(&a, 0, sizeof(a));
+  a.fp = p->fx;
+  a.tw = (const cpx<T>*)p->tw.at(p->ax.n);
+  a.nx = p->ax.n;
+  a.kx = p->Kx;
+  a.kxp = p->Kxp;
+  a.ndim = p->ndim;
+}
+
+// ---------------------------------------------------------------------------
+// transforms
+// ---------------------------------------------------------------------------
+
+template <typename T>
+int fwd_impl(ddx_plan* p, const T* x, cpx<T>* kc) {
+  int rc = ensure_ws(p, 1);
+  if (rc) return rc;
+  cpx<T>* A = (cpx<T>*)p->wsA;
+  cpx<T>* B = (cpx<T>*)p->wsB;
+  RowArgs<T> r;
+  row_defaults(p, r);
+  r.mode = ROW_R2C;
+  r.nin = 0; r.nout = 1;
+  r.xin[0] = x;
+  r.out[0] = A;
+  if ((rc = launch_row(p, r))) return rc;
+  ColArgs<T> c;
+  col_defaults(c);
+  geom_fy(p, c);
+  c.in[0] = A;
+  if (p->ndim == 2) {
+    c.out[0] = kc;
+    c.scale = T(1.0 / (double)p->npts);
+    return launch_col(p, c, 1, 1, -1, false);
+  }
+  c.out[0] = B;
+  if ((rc = launch_col(p, c, p->az.n, 1, -1, false))) return rc;
+  col_defaults(c);
+  geom_fz(p, c);
+  c.in[0] = B;
+  c.out[0] = kc;
+  c.scale = T(1.0 / (double)p->npts);
+  return launch_col(p, c, p->Ky, 1, -1, false);
+}
+
+template <typename T>
+int bwd_impl(ddx_plan* p, const cpx<T>* kc, T* x) {
+  int rc = ensure_ws(p, 1);
+  if (rc) return rc;
+  cpx<T>* A = (cpx<T>*)p->wsA;
+  cpx<T>* B = (cpx<T>*)p->wsB;
+  ColArgs<T> c;
+  const cpx<T>* ymix;
+  if (p->ndim == 3) {
+    col_defaults(c);
+    geom_bz(p, c);
+    c.in[0] = kc;
+    c.out[0] = A;
+    if ((rc = launch_col(p, c, p->Ky, 1, 1, false))) return rc;
+    col_defaults(c);
+    geom_by(p, c);
+    c.in[0] = A;
+    c.out[0] = B;
+    if ((rc = launch_col(p, c, p->az.n, 1, 1, false))) return rc;
+  } else {
+    col_defaults(c);
+    geom_by(p, c);
+    c.in[0] = kc;
+    c.out[0] = B;
+    if ((rc = launch_col(p, c, 1, 1, 1, false))) return rc;
+  }
+  ymix = B;
+  RowArgs<T> r;
+  row_defaults(p, r);
+  r.mode = ROW_C2R;
+  r.nin = 1; r.nout = 0;
+  r.in[0] = ymix;
+  r.xout[0] = x;
+  return launch_row(p, r);
+}
+
+// ---------------------------------------------------------------------------
+// fused RHS
+// ---------------------------------------------------------------------------
+
+struct PhysLayout {
+  int nstate;
+  int nq;  // derived x-space inputs
+  int np;  // x-space outputs
+  Recipe rec[DDX_MAXF];
+};
+
+Recipe one(int src, int dim = -1, double coef = 1.0) {
+  Recipe r;
+  memset(&r, 0, sizeof(r));
+  r.nterm = 1;
+  r.t[0].src = src; r.t[0].dim = dim; r.t[0].coef = coef;
+  return r;
+}
+
+Recipe two(int s0, int d0, double c0, int s1, int d1, double c1) {
+  Recipe r = one(s0, d0, c0);
+  r.nterm = 2;
+  r.t[1].src = s1; r.t[1].dim = d1; r.t[1].coef = c1;
+  return r;
+}
+
+// curl of the vector starting at state index b (3D): (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)
+void add_curl3(PhysLayout& L, int b) {
+  L.rec[L.nq++] = two(b + 2, 1, 1.0, b + 1, 2, -1.0);
+  L.rec[L.nq++] = two(b + 0, 2, 1.0, b + 2, 0, -1.0);
+  L.rec[L.nq++] = two(b + 1, 0, 1.0, b + 0, 1, -1.0);
+}
+
+int phys_layout(int physics, int nd, PhysLayout* L) {
+  memset(L, 0, sizeof(*L));
+  // velocity and vorticity
+  for (int i = 0; i < nd; ++i) L->rec[L->nq++] = one(i);
+  if (nd == 3) add_curl3(*L, 0);
+  else L->rec[L->nq++] = two(1, 0, 1.0, 0, 1, -1.0);
+  if (physics == DDX_HYDRO) {
+    L->nstate = nd;
+    L->np = nd;
+  } else if (physics == DDX_BOUSSINESQ) {
+    L->nstate = nd + 1;
+    L->rec[L->nq++] = one(nd);
+    for (int d = 0; d < nd; ++d) L->rec[L->nq++] = one(nd, d, 1.0);
+    L->np = nd + 1;
+  } else if (physics == DDX_MHD) {
+    L->nstate = 2 * nd;
+    for (int i = 0; i < nd; ++i) L->rec[L->nq++] = one(nd + i);
+    if (nd == 3) add_curl3(*L, nd);
+    else L->rec[L->nq++] = two(nd + 1, 0, 1.0, nd + 0, 1, -1.0);
+    L->np = (nd == 3) ? 6 : 3;
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+int nstate_of(const ddx_params* prm, int nd) {
+  PhysLayout L;
+  if (phys_layout(prm->physics, nd, &L)) return -1;
+  return L.nstate;
+}
+
+template <typename T>
+int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx<T>* const* deriv,
+             const cpx<T>* const* forcing) {
+  const int nd = p->ndim;
+  PhysLayout L;
+  if (phys_layout(prm->physics, nd, &L)) return fail(DDX_ERR_ARG, "unknown physics");
+  if (prm->physics == DDX_BOUSSINESQ && (prm->bdir < 0 || prm->bdir >= nd))
+    return fail(DDX_ERR_ARG, "boussinesq direction outside the grid's dimensions");
+  for (int s = 0; s < L.nstate; ++s)
+    if (!state[s] || !deriv[s]) return fail(DDX_ERR_ARG, "null state or deriv pointer");
+  int rc = ensure_ws(p, std::max(L.nq, L.np));
+  if (rc) return rc;
+  const long fs = ws_field_stride(p);
+  cpx<T>* A = (cpx<T>*)p->wsA;
+  cpx<T>* B = (cpx<T>*)p->wsB;
+
+  // backward: spectral prologue in the first column pass
+  ColArgs<T> c;
+  col_defaults(c);
+  for (int s = 0; s < L.nstate; ++s) c.src[s] = state[s];
+  for (int q = 0; q < L.nq; ++q) c.rec[q] = L.rec[q];
+  cpx<T>* mixed;  // (Nz, Ny, Kxp) per field
+  if (nd == 3) {
+    geom_bz(p, c);
+    for (int q = 0; q < L.nq; ++q) c.out[q] = A + q * fs;
+    if ((rc = launch_col(p, c, p->Ky, L.nq, 1, true))) return rc;
+    col_defaults(c);
+    geom_by(p, c);
+    for (int q = 0; q < L.nq; ++q) { c.in[q] = A + q * fs; c.out[q] = B + q * fs; }
+    if ((rc = launch_col(p, c, p->az.n, L.nq, 1, false))) return rc;
+  } else {
+    geom_by(p, c);
+    for (int q = 0; q < L.nq; ++q) c.out[q] = B + q * fs;
+    if ((rc = launch_col(p, c, 1, L.nq, 1, true))) return rc;
+  }
+  mixed = B;
+
+  // x rows: c2r, products, r2c
+  RowArgs<T> r;
+  row_defaults(p, r);
+  r.mode = ROW_PHYS;
+  r.physics = prm->physics;
+  r.nin = L.nq;
+  r.nout = L.np;
+  r.fpr_inv = T(1.0 / (4.0 * M_PI * (prm->physics == DDX_MHD ? prm->rho0 : 1.0)));
+  for (int q = 0; q < L.nq; ++q) r.in[q] = mixed + q * fs;
+  for (int o = 0; o < L.np; ++o) r.out[o] = A + o * fs;
+  if ((rc = launch_row(p, r))) return rc;
+
+  // forward column passes
+  cpx<T>* Uc;
+  col_defaults(c);
+  geom_fy(p, c);
+  for (int o = 0; o < L.np; ++o) c.in[o] = A + o * fs;
+  if (nd == 3) {
+    for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
+    if ((rc = launch_col(p, c, p->az.n, L.np, -1, false))) return rc;
+    col_defaults(c);
+    geom_fz(p, c);
+    for (int o = 0; o < L.np; ++o) { c.in[o] = B + o * fs; c.out[o] = A + o * fs; }
+    if ((rc = launch_col(p, c, p->Ky, L.np, -1, false))) return rc;
+    Uc = A;
+  } else {
+    for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
+    if ((rc = launch_col(p, c, 1, L.np, -1, false))) return rc;
+    Uc = B;
+  }
+
+  // k-space epilogue
+  EpiArgs<T> e;
+  memset(&e, 0, sizeof(e));
+  for (int o = 0; o < L.np; ++o) e.U[o] = Uc + o * fs;
+  for (int s = 0; s < L.nstate; ++s) {
+    e.state[s] = state[s];
+    e.deriv[s] = deriv[s];
+    e.forcing[s] = forcing ? forcing[s] : nullptr;
+  }
+  e.g = kgeom(p);
+  e.nk = p->nk;
+  e.physics = prm->physics;
+  e.ndim = nd;
+  e.bdir = prm->bdir;
+  e.s = T(1.0 / (double)p->npts);
+  e.galpha = T(prm->g * prm->alpha_t);
+  e.beta = T(prm->beta);
+  const int bs = 256;
+  hipLaunchKernelGGL((epilogue_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
+  LAUNCHCHK();
+  return 0;
+}
+
+template <typename T>
+int update_impl(ddx_plan* p, int kind, int ncomp, const cpx<T>* const* y, cpx<T>* const* out,
+                const cpx<T>* const* F1, const cpx<T>* const* F2, const double* coeff, int order,
+                double dt) {
+  if (ncomp < 1 || ncomp > DDX_MAXF) return fail(DDX_ERR_ARG, "bad component count");
+  if (kind < DDX_EULER || kind > DDX_ETD2RK2) return fail(DDX_ERR_ARG, "bad update kind");
+  UpdArgs<T> u;
+  memset(&u, 0, sizeof(u));
+  for (int i = 0; i < ncomp; ++i) {
+    if (!y[i] || !out[i] || !F1[i] || (kind >= DDX_ETD2RK1 && !F2[i]))
+      return fail(DDX_ERR_ARG, "null array in update");
+    u.y[i] = y[i];
+    u.out[i] = out[i];
+    u.F1[i] = F1[i];
+    u.F2[i] = F2 ? F2[i] : nullptr;
+    u.coeff[i] = coeff[i];
+  }
+  u.g = kgeom(p);
+  u.nk = p->nk;
+  u.ncomp = ncomp;
+  u.kind = kind;
+  u.order = order;
+  u.ndim = p->ndim;
+  u.dt = dt;
+  const int bs = 256;
+  hipLaunchKernelGGL((update_kernel<T>), dim3((p->nk + bs - 1) / bs, ncomp), dim3(bs), 0, p->stream, u);
+  LAUNCHCHK();
+  return 0;
+}
+
+void comp_coeffs(const ddx_params* prm, int nd, int nstate, double* coeff) {
+  for (int i = 0; i < nstate; ++i) coeff[i] = prm->nu;
+  if (prm->physics == DDX_BOUSSINESQ) coeff[nd] = prm->kappa;
+  if (prm->physics == DDX_MHD)
+    for (int i = nd; i < 2 * nd; ++i) coeff[i] = prm->eta;
+}
+
+template <typename T>
+int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const* y,
+              cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2, int nsteps, double dt) {
+  const int nd = p->ndim;
+  const int ns = nstate_of(prm, nd);
+  if (ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
+  if (integrator != DDX_RK2MID && integrator != DDX_RK2TRAP)
+    return fail(DDX_ERR_ARG, "unknown integrator");
+  for (int i = 0; i < ns; ++i) {
+    if (!y[i] || !d1[i] || !d2[i] || (integrator == DDX_RK2MID && !stage[i]))
+      return fail(DDX_ERR_ARG, "null buffer in step");
+  }
+  double coeff[DDX_MAXF];
+  comp_coeffs(prm, nd, ns, coeff);
+  const int vo = prm->viscosity_order;
+  int rc;
+  for (int s = 0; s < nsteps; ++s) {
+    if (integrator == DDX_RK2MID) {
+      // time_step.py:285-305
+      if ((rc = rhs_impl<T>(p, prm, y, d1, nullptr))) return rc;
+      if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, stage, d1, nullptr, coeff, vo, dt / 2.0))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, stage, d2, nullptr))) return rc;
+      if ((rc = update_impl<T>(p, DDX_ETD2RK2, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
+    } else {
+      // time_step.py:369-389
+      if ((rc = rhs_impl<T>(p, prm, y, d1, nullptr))) return rc;
+      if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, y, d1, nullptr, coeff, vo, dt))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d2, nullptr))) return rc;
+      if ((rc = update_impl<T>(p, DDX_ETD2RK1, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
+    }
+  }
+  return 0;
+}
+
+LayoutArgs layout_args(const ddx_plan* p) {
+  LayoutArgs L;
+  L.ndim = p->ndim;
+  L.nx = p->ax.n;
+  L.ny = p->ay.n;
+  L.nz = p->az.n;
+  L.Kx = p->Kx;
+  L.Kxp = p->Kxp;
+  L.Ky = p->Ky;
+  L.Kz = p->Kz;
+  L.cy = p->ay.c;
+  L.cz = p->az.c;
+  L.nk = p->nk;
+  L.nref = p->nref;
+  return L;
+}
+
+int check_plan(const ddx_plan* p) {
+  if (!p) return fail(DDX_ERR_ARG, "null plan");
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+const char* ddx_last_error(void) { return g_err.c_str(); }
+
+int ddx_version(void) { return 1; }
+
+int ddx_init(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(DDX_ERR_NODEV, "no HIP device available");
+  if (device < 0 || device >= n) return fail(DDX_ERR_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  g_device = device;
+  return 0;
+}
+
+int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
+                    ddx_plan** out) {
+  if (!out || !xshape || !length) return fail(DDX_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (g_device < 0) return fail(DDX_ERR_NODEV, "ddx_init has not been called");
+  if (ndim != 2 && ndim != 3) return fail(DDX_ERR_ARG, "Must use either 2 or 3 dimensions.");
+  if (precision != DDX_F64 && precision != DDX_F32) return fail(DDX_ERR_ARG, "bad precision");
+  if (precision == DDX_F32) return fail(DDX_ERR_UNSUPPORTED, "FP32 plans are not built yet");
+  for (int i = 0; i < ndim; ++i) {
+    if (xshape[i] < 1 || xshape[i] > (1 << 20)) return fail(DDX_ERR_ARG, "bad grid extent");
+    if (!(length[i] > 0.0)) return fail(DDX_ERR_ARG, "box lengths must be positive");
+  }
+  const int nx = (int)xshape[ndim - 1];
+  if (nx % 2) return fail(DDX_ERR_UNSUPPORTED, "x extent must be even");
+  ddx_plan* p = new ddx_plan();
+  p->device = g_device;
+  p->ndim = ndim;
+  p->precision = precision;
+  std::string why;
+  int ny = (int)xshape[ndim - 2], nz = ndim == 3 ? (int)xshape[0] : 1;
+  double lx = length[ndim - 1], ly = length[ndim - 2], lz = ndim == 3 ? length[0] : 1.0;
+  if (setup_axis(nx, lx, true, &p->ax, &why) || setup_axis(ny, ly, false, &p->ay, &why)) {
+    delete p;
+    return fail(DDX_ERR_UNSUPPORTED, why);
+  }
+  if (ndim == 3) {
+    if (setup_axis(nz, lz, false, &p->az, &why)) {
+      delete p;
+      return fail(DDX_ERR_UNSUPPORTED, why);
+    }
+  } else {
+    p->az.n = 1; p->az.L = 1.0; p->az.c = 0; p->az.K = 1; p->az.kfull.assign(1, 0.0);
+  }
+  if (make_fft_plan(p->ax.n, &p->fx) || make_fft_plan(p->ay.n, &p->fy) ||
+      make_fft_plan(p->az.n, &p->fz)) {
+    delete p;
+    return fail(DDX_ERR_UNSUPPORTED, "grid extents must factor into 2, 3 and 5");
+  }
+  p->Kx = p->ax.K;
+  p->Kxp = (p->Kx + 7) / 8 * 8;
+  p->Ky = p->ay.K;
+  p->Kz = p->az.K;
+  p->nk = (long)p->Ky * p->Kz * p->Kxp;
+  p->npts = (long)nx * ny * nz;
+  p->nref = (long)(nx / 2 + 1) * ny * nz;
+
+  // compact k tables
+  std::vector<double> hx(p->Kxp, 0.0), hy(p->Ky), hz(p->Kz);
+  for (int j = 0; j < p->Kx; ++j) hx[j] = p->ax.kfull[j];
+  for (int j = 0; j < p->Ky; ++j) hy[j] = p->ay.kfull[j <= p->ay.c ? j : j + p->ay.n - p->Ky];
+  for (int j = 0; j < p->Kz; ++j) hz[j] = p->az.kfull[j <= p->az.c ? j : j + p->az.n - p->Kz];
+  auto upload = [&](const std::vector<double>& h, double** d) -> int {
+    HIPCHK(hipMalloc((void**)d, h.size() * sizeof(double)));
+    p->owned.push_back(*d);
+    HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  };
+  int rc;
+  if ((rc = upload(hx, &p->dkx)) || (rc = upload(hy, &p->dky)) || (rc = upload(hz, &p->dkz))) {
+    ddx_plan_destroy(p);
+    return rc;
+  }
+  HIPCHK(hipStreamCreate(&p->own_stream));
+  p->stream = p->own_stream;
+  // FP64 twiddles generated on the device
+  const int ns[3] = {p->ax.n, p->ay.n, p->az.n};
+  for (int i = 0; i < 3; ++i) {
+    const int n = ns[i];
+    if (p->tw.count(n)) continue;
+    void* t = nullptr;
+    if (hipMalloc(&t, (size_t)n * 16) != hipSuccess) {
+      ddx_plan_destroy(p);
+      return fail(DDX_ERR_NOMEM, "twiddle allocation failed");
+    }
+    p->owned.push_back(t);
+    hipLaunchKernelGGL(twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, p->stream,
+                       (cpx<double>*)t, n);
+    p->tw[n] = t;
+  }
+  if (hipStreamSynchronize(p->stream) != hipSuccess) {
+    ddx_plan_destroy(p);
+    return fail(DDX_ERR_HIP, "plan setup failed");
+  }
+  *out = p;
+  return 0;
+}
+
+int ddx_plan_destroy(ddx_plan* p) {
+  if (!p) return 0;
+  if (p->own_stream) (void)hipStreamSynchronize(p->own_stream);
+  for (void* q : p->owned) (void)hipFree(q);
+  if (p->wsA) (void)hipFree(p->wsA);
+  if (p->wsB) (void)hipFree(p->wsB);
+  if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
+  delete p;
+  return 0;
+}
+
+int ddx_plan_kshape(const ddx_plan* p, int64_t* kref, int64_t* kc, int64_t* cut, int64_t* nkc) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  const int hx = p->ax.n / 2 + 1;
+  if (kref) {
+    if (p->ndim == 2) { kref[0] = hx; kref[1] = p->ay.n; kref[2] = 1; }
+    else { kref[0] = p->ay.n; kref[1] = p->az.n; kref[2] = hx; }
+  }
+  if (kc) { kc[0] = p->Ky; kc[1] = p->Kz; kc[2] = p->Kxp; }
+  if (cut) { cut[0] = p->ax.c; cut[1] = p->ay.c; cut[2] = p->az.c; }
+  if (nkc) *nkc = p->nk;
+  return 0;
+}
+
+int ddx_plan_ktable(const ddx_plan* p, int axis, double* out, int64_t n) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  const AxisInfo* a = axis == 0 ? &p->ax : axis == 1 ? &p->ay : axis == 2 ? &p->az : nullptr;
+  if (!a || !out) return fail(DDX_ERR_ARG, "bad axis");
+  if (n != a->K) return fail(DDX_ERR_ARG, "ktable length mismatch");
+  for (int j = 0; j < a->K; ++j) out[j] = a->kfull[(axis == 0 || j <= a->c) ? j : j + a->n - a->K];
+  return 0;
+}
+
+int ddx_plan_set_stream(ddx_plan* p, void* stream) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  p->stream = stream ? (hipStream_t)stream : p->own_stream;
+  return 0;
+}
+
+int ddx_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(DDX_ERR_ARG, "null pointer");
+  if (g_device < 0) return fail(DDX_ERR_NODEV, "ddx_init has not been called");
+  *ptr = nullptr;
+  if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) return fail(DDX_ERR_NOMEM, "hipMalloc failed");
+  return 0;
+}
+
+int ddx_free(void* ptr) {
+  if (ptr) HIPCHK(hipFree(ptr));
+  return 0;
+}
+
+int ddx_h2d(ddx_plan* p, void* dst, const void* src, size_t bytes) {
+  if (check_plan(p) || !dst || !src) return fail(DDX_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int ddx_d2h(ddx_plan* p, void* dst, const void* src, size_t bytes) {
+  if (check_plan(p) || !dst || !src) return fail(DDX_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int ddx_zero(ddx_plan* p, void* dst, size_t bytes) {
+  if (check_plan(p) || !dst) return fail(DDX_ERR_ARG, "null argument");
+  if (bytes == 0) return 0;
+  if (bytes % 4 == 0 && ((uintptr_t)dst % 4) == 0) {
+    const size_t nw = bytes / 4;
+    const size_t blocks = std::min<size_t>((nw + 255) / 256, 65536);
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, p->stream, (uint32_t*)dst, nw);
+  } else {
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3((bytes + 255) / 256), dim3(256), 0, p->stream,
+                       (uint8_t*)dst, bytes);
+  }
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_sync(ddx_plan* p) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int ddx_pack(ddx_plan* p, const void* kref, void* kc) {
+  if (check_plan(p) || !kref || !kc) return fail(DDX_ERR_ARG, "null argument");
+  LayoutArgs L = layout_args(p);
+  hipLaunchKernelGGL((pack_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, L,
+                     (const cpx<double>*)kref, (cpx<double>*)kc);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_unpack(ddx_plan* p, const void* kc, void* kref) {
+  if (check_plan(p) || !kref || !kc) return fail(DDX_ERR_ARG, "null argument");
+  LayoutArgs L = layout_args(p);
+  hipLaunchKernelGGL((unpack_kernel<double>), dim3((p->nref + 255) / 256), dim3(256), 0, p->stream,
+                     L, (const cpx<double>*)kc, (cpx<double>*)kref);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_fwd(ddx_plan* p, const void* x, void* kc) {
+  if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
+  return fwd_impl<double>(p, (const double*)x, (cpx<double>*)kc);
+}
+
+int ddx_bwd(ddx_plan* p, const void* kc, void* x) {
+  if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
+  return bwd_impl<double>(p, (const cpx<double>*)kc, (double*)x);
+}
+
+int ddx_deriv(ddx_plan* p, const void* kc, void* out, int dim) {
+  if (check_plan(p) || !kc || !out) return fail(DDX_ERR_ARG, "null argument");
+  if (dim < 0 || dim >= p->ndim) return fail(DDX_ERR_ARG, "bad derivative dimension");
+  hipLaunchKernelGGL((deriv_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream,
+                     kgeom(p), p->nk, dim, (const cpx<double>*)kc, (cpx<double>*)out);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_project(ddx_plan* p, void* const* comps) {
+  if (check_plan(p) || !comps) return fail(DDX_ERR_ARG, "null argument");
+  ProjArgs<double> a;
+  memset(&a, 0, sizeof(a));
+  for (int i = 0; i < p->ndim; ++i) {
+    if (!comps[i]) return fail(DDX_ERR_ARG, "null component");
+    a.c[i] = (cpx<double>*)comps[i];
+  }
+  a.g = kgeom(p);
+  a.nk = p->nk;
+  a.ndim = p->ndim;
+  hipLaunchKernelGGL((project_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_update(ddx_plan* p, int kind, const void* y, void* out, const void* F1, const void* F2,
+               double coeff, int order, double dt) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  const cpx<double>* yy = (const cpx<double>*)y;
+  cpx<double>* oo = (cpx<double>*)out;
+  const cpx<double>* f1 = (const cpx<double>*)F1;
+  const cpx<double>* f2 = (const cpx<double>*)F2;
+  return update_impl<double>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, order, dt);
+}
+
+int ddx_rhs(ddx_plan* p, const ddx_params* prm, const void* const* state, void* const* deriv,
+            const void* const* forcing) {
+  if (check_plan(p) || !prm || !state || !deriv) return fail(DDX_ERR_ARG, "null argument");
+  return rhs_impl<double>(p, prm, (const cpx<double>* const*)state, (cpx<double>* const*)deriv,
+                          (const cpx<double>* const*)forcing);
+}
+
+int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
+             void* const* stage, void* const* deriv1, void* const* deriv2, int nsteps, double dt) {
+  if (check_plan(p) || !prm || !state || !deriv1 || !deriv2) return fail(DDX_ERR_ARG, "null argument");
+  if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
+  if (nsteps < 0) return fail(DDX_ERR_ARG, "negative step count");
+  void* const* st = stage ? stage : state;
+  return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
+                           (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2, nsteps, dt);
+}
+
+}  // extern "C"

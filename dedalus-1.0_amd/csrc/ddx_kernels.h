@@ -1,0 +1,531 @@
+// Device kernels of libddx: column FFT passes, x-row passes fused with the
+// physics products, k-space epilogues (projection, curl, buoyancy), ETD
+// updates, layout conversion.
+//
+// Data layout in HBM
+//   x space   : (Nz, Ny, Nx) real, x fastest (Nz = 1 in 2D)
+//   compact k : (Ky, Kz, Kxp) complex, kx fastest, only 2/3-kept modes;
+//               row index of a full axis j <-> frequency j (j <= c) or
+//               j + N - K (j > c); pad columns kx >= Kx are zero
+//   mixed     : (Ky, Nz, Kxp) after the z pass, (Nz, Ny, Kxp) after the y pass
+#pragma once
+
+#include "fft_core.h"
+
+namespace ddx {
+
+#define DDX_MAXF 16
+
+// One term of a derived field: coef * (i k_dim)^[dim >= 0] * source[src]
+struct Term {
+  int src;
+  int dim;  // -1: none, 0: x, 1: y, 2: z
+  double coef;
+};
+struct Recipe {
+  int nterm;
+  Term t[2];
+};
+
+// ---------------------------------------------------------------------------
+// column pass: batched FFT along a strided axis, W columns per workgroup
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct ColArgs {
+  const cpx<T>* in[DDX_MAXF];
+  cpx<T>* out[DDX_MAXF];
+  const cpx<T>* src[DDX_MAXF];  // sources for the spectral prologue
+  Recipe rec[DDX_MAXF];
+  FftPlan fp;
+  const cpx<T>* tw;
+  const double* k_row;   // compact wavenumbers along the row axis (prologue)
+  const double* k_slab;  // compact wavenumbers along the slab axis (prologue)
+  const double* k_col;   // kx table (prologue)
+  long in_rs, in_ss, out_rs, out_ss;  // row / slab strides (complex elements)
+  int n;                 // FFT length
+  int in_rows, in_c;     // in_c < 0: rows are the full axis
+  int out_rows, out_c;   // out_c < 0: rows are the full axis
+  int ncols;             // columns per slab
+  int W;                 // tile width (columns per workgroup)
+  int row_dim, slab_dim; // physical axis (0 x, 1 y, 2 z) of rows / slabs, -1 none
+  T scale;
+};
+
+template <typename T, int D, bool SPECTRAL>
+__global__ void __launch_bounds__(256) col_fft_kernel(const ColArgs<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int W = a.W, n = a.n;
+  cpx<T>* B = A + n * W;
+  const int f = blockIdx.z;
+  const long slab = blockIdx.y;
+  const int c0 = blockIdx.x * W;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+
+  for (int e = tid; e < n * W; e += nthr) {
+    const int m = e / W, c = e - m * W;
+    const int col = c0 + c;
+    cpx<T> v = mk(T(0), T(0));
+    if (col < a.ncols) {
+      int row;
+      if (a.in_c < 0) row = m;
+      else if (m <= a.in_c) row = m;
+      else if (m >= n - a.in_c) row = m - (n - a.in_rows);
+      else row = -1;
+      if (row >= 0) {
+        const long off = slab * a.in_ss + (long)row * a.in_rs + col;
+        if (SPECTRAL) {
+          const int nterm = a.rec[f].nterm;
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            if (t < nterm) {
+              const Term tm = a.rec[f].t[t];
+              cpx<T> x = a.src[tm.src][off];
+              if (tm.dim >= 0) {
+                const double kk = (tm.dim == 0) ? a.k_col[col]
+                                  : (tm.dim == a.row_dim ? a.k_row[row] : a.k_slab[slab]);
+                x = mul_is(x, T(kk));
+              }
+              v = v + scale(x, T(tm.coef));
+            }
+          }
+        } else {
+          v = a.in[f][off];
+        }
+      }
+    }
+    A[m * W + c] = v;
+  }
+  __syncthreads();
+  cpx<T>* R = lds_fft<T, T, D>(A, B, a.fp, W, W, 1, true, a.tw, tid, nthr);
+  for (int e = tid; e < a.out_rows * W; e += nthr) {
+    const int j = e / W, c = e - j * W;
+    const int col = c0 + c;
+    if (col >= a.ncols) continue;
+    const int m = (a.out_c < 0 || j <= a.out_c) ? j : j + (n - a.out_rows);
+    a.out[f][slab * a.out_ss + (long)j * a.out_rs + col] = scale(R[m * W + c], a.scale);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x-row pass: c2r of complex half-spectrum rows (two real rows per complex
+// FFT), x-space physics, r2c back to half-spectrum rows
+// ---------------------------------------------------------------------------
+
+enum { ROW_C2R = 0, ROW_R2C = 1, ROW_PHYS = 2 };
+enum { PH_HYDRO = 0, PH_BOUSS = 1, PH_MHD = 2 };
+
+template <typename T>
+struct RowArgs {
+  const cpx<T>* in[DDX_MAXF];
+  cpx<T>* out[DDX_MAXF];
+  const T* xin[DDX_MAXF];
+  T* xout[DDX_MAXF];
+  FftPlan fp;
+  const cpx<T>* tw;
+  int nin, nout;
+  int nx, kx, kxp;
+  int mode, physics, ndim;
+  T fpr_inv;  // 1 / (4 pi rho0)
+};
+
+// x-space products.  Inputs per physics (ndim 3 / 2):
+//   HYDRO : u[3], w[3]            | u[2], w
+//   BOUSS : u[3], w[3], T, dT[3]  | u[2], w, T, dT[2]
+//   MHD   : u[3], w[3], B[3], J[3]| u[2], w, B[2], J
+// Outputs: velocity u x w (+ J x B / (4 pi rho0)); BOUSS adds -u.grad T;
+// MHD adds the electric field u x B (3D) or its z component (2D).
+template <typename T>
+__device__ inline void row_physics(int physics, int ndim, T fi, const T* q, T* o) {
+  if (ndim == 3) {
+    const T u0 = q[0], u1 = q[1], u2 = q[2], w0 = q[3], w1 = q[4], w2 = q[5];
+    o[0] = u1 * w2 - u2 * w1;
+    o[1] = u2 * w0 - u0 * w2;
+    o[2] = u0 * w1 - u1 * w0;
+    if (physics == PH_BOUSS) {
+      o[3] = -(u0 * q[7] + u1 * q[8] + u2 * q[9]);
+    } else if (physics == PH_MHD) {
+      const T b0 = q[6], b1 = q[7], b2 = q[8], j0 = q[9], j1 = q[10], j2 = q[11];
+      o[0] += fi * (j1 * b2 - j2 * b1);
+      o[1] += fi * (j2 * b0 - j0 * b2);
+      o[2] += fi * (j0 * b1 - j1 * b0);
+      o[3] = u1 * b2 - u2 * b1;
+      o[4] = u2 * b0 - u0 * b2;
+      o[5] = u0 * b1 - u1 * b0;
+    }
+  } else {
+    const T u0 = q[0], u1 = q[1], w = q[2];
+    o[0] = u1 * w;
+    o[1] = -u0 * w;
+    if (physics == PH_BOUSS) {
+      o[2] = -(u0 * q[4] + u1 * q[5]);
+    } else if (physics == PH_MHD) {
+      const T b0 = q[3], b1 = q[4], j = q[5];
+      o[0] -= fi * j * b1;
+      o[1] += fi * j * b0;
+      o[2] = u0 * b1 - u1 * b0;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) row_kernel(const RowArgs<T> a) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nx = a.nx;
+  const int nsin = (a.nin + 1) / 2, nsout = (a.nout + 1) / 2;
+  const int nseq = nsin > nsout ? nsin : nsout;
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* B = A + nseq * nx;
+  const long r = blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int cx = a.kx - 1;
+  cpx<T>* X;
+
+  if (a.mode == ROW_R2C) {
+    for (int e = tid; e < nsout * nx; e += nthr) {
+      const int q = e / nx, m = e - q * nx;
+      T re = a.xin[2 * q][r * nx + m];
+      T im = (2 * q + 1 < a.nout) ? a.xin[2 * q + 1][r * nx + m] : T(0);
+      A[q * nx + m] = mk(re, im);
+    }
+    __syncthreads();
+    X = A;
+  } else {
+    // Z[k] = A[k] + i B[k] over the full spectrum from the half spectra
+    for (int e = tid; e < nsin * nx; e += nthr) {
+      const int q = e / nx, m = e - q * nx;
+      const bool hasb = (2 * q + 1 < a.nin);
+      cpx<T> va = mk(T(0), T(0)), vb = mk(T(0), T(0));
+      if (m <= cx) {
+        va = a.in[2 * q][r * a.kxp + m];
+        if (hasb) vb = a.in[2 * q + 1][r * a.kxp + m];
+        if (m == 0) { va.y = T(0); vb.y = T(0); }  // c2r keeps the real part of kx = 0
+      } else if (m >= nx - cx) {
+        va = conj(a.in[2 * q][r * a.kxp + (nx - m)]);
+        if (hasb) vb = conj(a.in[2 * q + 1][r * a.kxp + (nx - m)]);
+      }
+      A[q * nx + m] = mk(va.x - vb.y, va.y + vb.x);
+    }
+    __syncthreads();
+    X = lds_fft<T, T, 1>(A, B, a.fp, nsin, 1, nx, false, a.tw, tid, nthr);
+    if (a.mode == ROW_C2R) {
+      for (int e = tid; e < a.nin * nx; e += nthr) {
+        const int f = e / nx, m = e - f * nx;
+        const cpx<T> z = X[(f >> 1) * nx + m];
+        a.xout[f][r * nx + m] = (f & 1) ? z.y : z.x;
+      }
+      return;
+    }
+    // physics on the real values, result packed into the other buffer
+    cpx<T>* Y = (X == A) ? B : A;
+    for (int m = tid; m < nx; m += nthr) {
+      T q[12], o[6];
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        cpx<T> z = mk(T(0), T(0));
+        if (s < nsin) z = X[s * nx + m];
+        q[2 * s] = z.x;
+        q[2 * s + 1] = z.y;
+      }
+#pragma unroll
+      for (int f = 0; f < 6; ++f) o[f] = T(0);
+      row_physics<T>(a.physics, a.ndim, a.fpr_inv, q, o);
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+        if (s < nsout) Y[s * nx + m] = mk(o[2 * s], o[2 * s + 1]);
+    }
+    __syncthreads();
+    X = Y;
+  }
+  cpx<T>* Bf = (X == A) ? B : A;
+  cpx<T>* Z = lds_fft<T, T, -1>(X, Bf, a.fp, nsout, 1, nx, false, a.tw, tid, nthr);
+  for (int e = tid; e < nsout * a.kxp; e += nthr) {
+    const int q = e / a.kxp, k = e - q * a.kxp;
+    cpx<T> ra = mk(T(0), T(0)), rb = mk(T(0), T(0));
+    if (k < a.kx) {
+      const cpx<T> zk = Z[q * nx + k];
+      const cpx<T> zm = conj(Z[q * nx + ((nx - k) % nx)]);
+      const cpx<T> s = zk + zm, d = zk - zm;
+      ra = mk(T(0.5) * s.x, T(0.5) * s.y);
+      rb = mk(T(0.5) * d.y, T(-0.5) * d.x);
+    }
+    a.out[2 * q][r * a.kxp + k] = ra;
+    if (2 * q + 1 < a.nout) a.out[2 * q + 1][r * a.kxp + k] = rb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// compact-k pointwise kernels
+// ---------------------------------------------------------------------------
+
+struct KGeom {
+  int Ky, Kz, Kx, Kxp;
+  const double* kx;
+  const double* ky;
+  const double* kz;
+};
+
+template <typename T>
+struct EpiArgs {
+  const cpx<T>* U[8];            // forward-transformed x-space products (unscaled)
+  const cpx<T>* state[DDX_MAXF];
+  const cpx<T>* forcing[DDX_MAXF];
+  cpx<T>* deriv[DDX_MAXF];
+  KGeom g;
+  long nk;
+  int physics, ndim, bdir;
+  T s;        // 1 / N
+  T galpha;   // g * alpha_t
+  T beta;
+};
+
+template <typename T>
+__device__ inline void kvec(const KGeom& g, long idx, int& col, T& kx, T& ky, T& kz) {
+  col = (int)(idx % g.Kxp);
+  const long rs = idx / g.Kxp;
+  const int row = (int)(rs % g.Kz);
+  const int slab = (int)(rs / g.Kz);
+  kx = T(g.kx[col]);
+  ky = T(g.ky[slab]);
+  kz = T(g.kz[row]);
+}
+
+template <typename T>
+__global__ void epilogue_kernel(const EpiArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nk) return;
+  int col;
+  T kx, ky, kz;
+  kvec<T>(a.g, idx, col, kx, ky, kz);
+  const int nd = a.ndim;
+  const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
+  if (col >= a.g.Kx) {
+    for (int v = 0; v < nvar; ++v) a.deriv[v][idx] = mk(T(0), T(0));
+    return;
+  }
+  const T kk[3] = {kx, ky, kz};
+  cpx<T> F[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    F[i] = mk(T(0), T(0));
+    if (i < nd) {
+      F[i] = scale(a.U[i][idx], a.s);
+      if (a.forcing[i]) F[i] = F[i] + a.forcing[i][idx];
+      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(a.state[nd][idx], a.galpha);
+    }
+  }
+  // projection F <- F - k (k.F) / |k|^2, |k|^2 = 0 replaced by 1
+  T k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 == T(0)) k2 = T(1);
+  cpx<T> kf = mk(T(0), T(0));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) kf = kf + scale(F[i], kk[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < nd) a.deriv[i][idx] = F[i] - scale(kf, kk[i] / k2);
+  if (a.physics == PH_BOUSS) {
+    cpx<T> ft = scale(a.U[nd][idx], a.s) - scale(a.state[a.bdir][idx], a.beta);
+    if (a.forcing[nd]) ft = ft + a.forcing[nd][idx];
+    a.deriv[nd][idx] = ft;
+  } else if (a.physics == PH_MHD) {
+    if (nd == 3) {
+      const cpx<T> e0 = scale(a.U[3][idx], a.s), e1 = scale(a.U[4][idx], a.s),
+                   e2 = scale(a.U[5][idx], a.s);
+      // i k x E
+      a.deriv[3][idx] = mul_is(e2, ky) - mul_is(e1, kz);
+      a.deriv[4][idx] = mul_is(e0, kz) - mul_is(e2, kx);
+      a.deriv[5][idx] = mul_is(e1, kx) - mul_is(e0, ky);
+    } else {
+      const cpx<T> e = scale(a.U[2][idx], a.s);
+      a.deriv[2][idx] = mul_is(e, ky);
+      a.deriv[3][idx] = mk(T(0), T(0)) - mul_is(e, kx);
+    }
+  }
+}
+
+// phi functions with the reference's branches (forward_step_cy_2d.pyx:123-148,
+// forward_step_cy_3d.pyx:54-59): Taylor series to degree 14 for |Z| < 0.5
+// (phi0 by Taylor in 2D, by exp in 3D), closed forms otherwise.
+__device__ inline double taylor_phi(int k, double z) {
+  double factj = 1.0, sum = 0.0, zp = 1.0;
+  for (int j = k; j < 15; ++j) {
+    if (j > 1) factj *= j;
+    sum += zp / factj;
+    zp *= z;
+  }
+  return sum;
+}
+
+template <typename T>
+struct UpdArgs {
+  const cpx<T>* y[DDX_MAXF];
+  cpx<T>* out[DDX_MAXF];
+  const cpx<T>* F1[DDX_MAXF];
+  const cpx<T>* F2[DDX_MAXF];
+  double coeff[DDX_MAXF];
+  KGeom g;
+  long nk;
+  int ncomp, kind, order, ndim;
+  double dt;
+};
+
+enum { UPD_EULER = 0, UPD_ETD1 = 1, UPD_ETD2RK1 = 2, UPD_ETD2RK2 = 3 };
+
+template <typename T>
+__global__ void update_kernel(const UpdArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int comp = blockIdx.y;
+  if (idx >= a.nk || comp >= a.ncomp) return;
+  int col;
+  double kx, ky, kz;
+  kvec<double>(a.g, idx, col, kx, ky, kz);
+  if (col >= a.g.Kx) {
+    a.out[comp][idx] = mk(T(0), T(0));
+    return;
+  }
+  const double dt = a.dt;
+  const cpx<T> y = a.y[comp][idx];
+  const cpx<T> f1 = a.F1[comp][idx];
+  const cpx<T> f2 = (a.kind >= UPD_ETD2RK1) ? a.F2[comp][idx] : mk(T(0), T(0));
+  double Z = 0.0;
+  const double c = a.coeff[comp];
+  if (c != 0.0 && a.kind != UPD_EULER) {
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    double kp;
+    if (a.order == 1) kp = k2;
+    else if (a.order == 2) kp = k2 * k2;
+    else kp = pow(k2, (double)a.order);
+    Z = (-(c * kp)) * dt;
+  }
+  cpx<T> res;
+  if (Z == 0.0) {
+    if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, T(dt / 2.0));
+    else if (a.kind == UPD_ETD2RK2) res = y + scale(f2, T(dt));
+    else res = y + scale(f1, T(dt));
+  } else {
+    double p0, p1, p2;
+    if (Z < 0.5 && Z > -0.5) {
+      p0 = (a.ndim == 2) ? taylor_phi(0, Z) : exp(Z);
+      p1 = taylor_phi(1, Z);
+      p2 = taylor_phi(2, Z);
+    } else {
+      p0 = exp(Z);
+      p1 = (p0 - 1.0) / Z;
+      p2 = (p1 - 1.0) / Z;
+    }
+    if (a.kind == UPD_ETD1) res = scale(y, T(p0)) + scale(f1, T(p1 * dt));
+    else if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, T(p2 * dt));
+    else res = scale(y, T(p0)) + scale(f2 - f1, T(2.0 * p2 * dt)) + scale(f1, T(p1 * dt));
+  }
+  a.out[comp][idx] = res;
+}
+
+// layout conversion: kref (reference transposed half spectrum) <-> compact
+struct LayoutArgs {
+  int ndim;
+  int nx, ny, nz;       // x-space extents
+  int Kx, Kxp, Ky, Kz;
+  int cy, cz;
+  long nk;              // compact elements
+  long nref;            // reference elements
+};
+
+__device__ inline long ref_index(const LayoutArgs& L, int m, int fy, int fz) {
+  const int hx = L.nx / 2 + 1;
+  if (L.ndim == 2) return (long)m * L.ny + fy;
+  return ((long)fy * L.nz + fz) * hx + m;
+}
+
+template <typename T>
+__global__ void pack_kernel(const LayoutArgs L, const cpx<T>* ref, cpx<T>* kc) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L.nk) return;
+  const int col = (int)(idx % L.Kxp);
+  const long rs = idx / L.Kxp;
+  const int row = (int)(rs % L.Kz);
+  const int slab = (int)(rs / L.Kz);
+  if (col >= L.Kx) { kc[idx] = mk(T(0), T(0)); return; }
+  const int fy = slab <= L.cy ? slab : slab + (L.ny - L.Ky);
+  const int fz = row <= L.cz ? row : row + (L.nz - L.Kz);
+  kc[idx] = ref[ref_index(L, col, fy, fz)];
+}
+
+template <typename T>
+__global__ void unpack_kernel(const LayoutArgs L, const cpx<T>* kc, cpx<T>* ref) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L.nref) return;
+  const int hx = L.nx / 2 + 1;
+  int m, fy, fz;
+  if (L.ndim == 2) { m = (int)(idx / L.ny); fy = (int)(idx % L.ny); fz = 0; }
+  else { m = (int)(idx % hx); long t = idx / hx; fz = (int)(t % L.nz); fy = (int)(t / L.nz); }
+  cpx<T> v = mk(T(0), T(0));
+  int row = -1, slab = -1;
+  if (fy <= L.cy) slab = fy; else if (fy >= L.ny - L.cy) slab = fy - (L.ny - L.Ky);
+  if (fz <= L.cz) row = fz; else if (fz >= L.nz - L.cz) row = fz - (L.nz - L.Kz);
+  if (m < L.Kx && slab >= 0 && row >= 0) v = kc[((long)slab * L.Kz + row) * L.Kxp + m];
+  ref[idx] = v;
+}
+
+template <typename T>
+__global__ void deriv_kernel(KGeom g, long nk, int dim, const cpx<T>* in, cpx<T>* out) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nk) return;
+  int col;
+  T kx, ky, kz;
+  kvec<T>(g, idx, col, kx, ky, kz);
+  if (col >= g.Kx) { out[idx] = mk(T(0), T(0)); return; }
+  const T k = dim == 0 ? kx : (dim == 1 ? ky : kz);
+  out[idx] = mul_is(in[idx], k);
+}
+
+template <typename T>
+struct ProjArgs {
+  cpx<T>* c[3];
+  KGeom g;
+  long nk;
+  int ndim;
+};
+
+template <typename T>
+__global__ void project_kernel(const ProjArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nk) return;
+  int col;
+  T kx, ky, kz;
+  kvec<T>(a.g, idx, col, kx, ky, kz);
+  if (col >= a.g.Kx) return;
+  const T kk[3] = {kx, ky, kz};
+  T k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 == T(0)) k2 = T(1);
+  cpx<T> kf = mk(T(0), T(0));
+  for (int i = 0; i < a.ndim; ++i) kf = kf + scale(a.c[i][idx], kk[i]);
+  for (int i = 0; i < a.ndim; ++i) a.c[i][idx] = a.c[i][idx] - scale(kf, kk[i] / k2);
+}
+
+__global__ void twiddle_kernel(cpx<double>* tw, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s, c;
+  sincospi(2.0 * (double)j / (double)n, &s, &c);
+  tw[j] = mk(c, -s);
+}
+
+__global__ void twiddle_f32_kernel(const cpx<double>* tw64, cpx<float>* tw32, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  tw32[j] = mk(float(tw64[j].x), float(tw64[j].y));
+}
+
+__global__ void zero_kernel(uint32_t* p, size_t nwords) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+__global__ void zero_bytes_kernel(uint8_t* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace ddx

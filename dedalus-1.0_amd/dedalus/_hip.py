@@ -1,0 +1,310 @@
+"""ctypes binding to libddx.so, the MI355X pseudospectral step.
+
+The library is built in-tree (``build_ddx.py`` next to this package) and
+loaded from this directory.  There is no CPU fallback: every entry point
+raises ``DDXError`` when the library or a HIP device is missing.
+
+Device buffers are owned by ``DevBuf`` objects (freed when collected); the
+library never frees caller memory.  ``Plan`` wraps one grid (shape, box
+lengths) and owns its stream and workspace.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libddx.so")
+
+# enums of include/ddx.h
+F64, F32 = 0, 1
+HYDRO, BOUSSINESQ, MHD = 0, 1, 2
+RK2MID, RK2TRAP = 0, 1
+EULER, ETD1, ETD2RK1, ETD2RK2 = 0, 1, 2, 3
+
+ERR_NAMES = {-1: "DDX_ERR_ARG", -2: "DDX_ERR_HIP", -3: "DDX_ERR_UNSUPPORTED",
+             -4: "DDX_ERR_NOMEM", -5: "DDX_ERR_NODEV"}
+
+
+class DDXError(RuntimeError):
+    """Failure reported by libddx (or the library is missing)."""
+
+    def __init__(self, code, msg):
+        self.code = code
+        RuntimeError.__init__(self, "%s: %s" % (ERR_NAMES.get(code, code), msg))
+
+
+class Params(ctypes.Structure):
+    """Mirror of ddx_params."""
+    _fields_ = [("physics", ctypes.c_int),
+                ("viscosity_order", ctypes.c_int),
+                ("nu", ctypes.c_double),
+                ("kappa", ctypes.c_double),
+                ("eta", ctypes.c_double),
+                ("g", ctypes.c_double),
+                ("alpha_t", ctypes.c_double),
+                ("beta", ctypes.c_double),
+                ("bdir", ctypes.c_int),
+                ("rho0", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); every exported symbol of include/ddx.h
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+SIGNATURES = {
+    "ddx_init": (_I, [_I]),
+    "ddx_last_error": (ctypes.c_char_p, []),
+    "ddx_version": (_I, []),
+    "ddx_plan_create": (_I, [_I, _I64P, _DP, _I, _PP]),
+    "ddx_plan_destroy": (_I, [_VP]),
+    "ddx_plan_kshape": (_I, [_VP, _I64P, _I64P, _I64P, _I64P]),
+    "ddx_plan_ktable": (_I, [_VP, _I, _DP, ctypes.c_int64]),
+    "ddx_plan_set_stream": (_I, [_VP, _VP]),
+    "ddx_alloc": (_I, [_SZ, _PP]),
+    "ddx_free": (_I, [_VP]),
+    "ddx_h2d": (_I, [_VP, _VP, _VP, _SZ]),
+    "ddx_d2h": (_I, [_VP, _VP, _VP, _SZ]),
+    "ddx_zero": (_I, [_VP, _VP, _SZ]),
+    "ddx_sync": (_I, [_VP]),
+    "ddx_pack": (_I, [_VP, _VP, _VP]),
+    "ddx_unpack": (_I, [_VP, _VP, _VP]),
+    "ddx_fwd": (_I, [_VP, _VP, _VP]),
+    "ddx_bwd": (_I, [_VP, _VP, _VP]),
+    "ddx_deriv": (_I, [_VP, _VP, _VP, _I]),
+    "ddx_project": (_I, [_VP, _PP]),
+    "ddx_update": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _D, _I, _D]),
+    "ddx_rhs": (_I, [_VP, ctypes.POINTER(Params), _PP, _PP, _PP]),
+    "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _I, _D]),
+}
+
+_lib = None
+_lock = threading.Lock()
+_device = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libddx.so and declare its signatures (no device needed)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise DDXError(-5, "libddx.so is not built (%s); run build_ddx.py" % path)
+            lib = ctypes.CDLL(path)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise DDXError(rc, _lib.ddx_last_error().decode())
+
+
+def init(device=0):
+    """Select the HIP device once per process."""
+    global _device
+    lib = load_library()
+    if _device is None:
+        _check(lib.ddx_init(int(device)))
+        _device = int(device)
+    return _device
+
+
+def ptr_array(ptrs):
+    """ctypes void* array from ints / None."""
+    arr = (ctypes.c_void_p * max(len(ptrs), 1))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr
+
+
+class DevBuf:
+    """Device allocation of nbytes (zero-filled on request)."""
+
+    def __init__(self, nbytes):
+        init()
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(_lib.ddx_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.ddx_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __int__(self):
+        return self.ptr
+
+
+class Plan:
+    """One grid: xshape in reference order, box lengths, precision."""
+
+    def __init__(self, shape, length, precision=F64):
+        init()
+        self.shape = tuple(int(n) for n in shape)
+        self.length = tuple(float(x) for x in length)
+        self.ndim = len(self.shape)
+        self.precision = precision
+        sh = (ctypes.c_int64 * self.ndim)(*self.shape)
+        ln = (ctypes.c_double * self.ndim)(*self.length)
+        p = ctypes.c_void_p()
+        _check(_lib.ddx_plan_create(self.ndim, sh, ln, precision, ctypes.byref(p)))
+        self.handle = p
+        kref = (ctypes.c_int64 * 3)()
+        kc = (ctypes.c_int64 * 3)()
+        cut = (ctypes.c_int64 * 3)()
+        nkc = ctypes.c_int64()
+        _check(_lib.ddx_plan_kshape(p, kref, kc, cut, ctypes.byref(nkc)))
+        self.kref_shape = tuple(kref[i] for i in range(2 if self.ndim == 2 else 3))
+        self.kc_shape = tuple(kc[i] for i in range(3))
+        self.cut = tuple(cut[i] for i in range(3))
+        self.nkc = nkc.value
+        self.npts = int(np.prod(self.shape))
+        self.nref = int(np.prod(self.kref_shape))
+        self.cbytes = 16 if precision == F64 else 8
+        self.rbytes = 8 if precision == F64 else 4
+        self.ktables = []
+        for ax, n in ((0, self.cut[0] + 1), (1, 2 * self.cut[1] + 1),
+                      (2, 2 * self.cut[2] + 1)):
+            buf = np.zeros(n, dtype=np.float64)
+            _check(_lib.ddx_plan_ktable(p, ax, buf.ctypes.data_as(_DP), n))
+            self.ktables.append(buf)
+        self._stage = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.ddx_plan_destroy(h)
+            self.handle = None
+
+    # --- buffers --------------------------------------------------------
+    def alloc_k(self, zero=True):
+        b = DevBuf(self.nkc * self.cbytes)
+        if zero:
+            self.zero(b)
+        return b
+
+    def alloc_x(self, zero=True):
+        b = DevBuf(self.npts * self.rbytes)
+        if zero:
+            self.zero(b)
+        return b
+
+    def staging(self):
+        """Device buffer for one reference-layout k array (layout conversion)."""
+        if self._stage is None:
+            self._stage = DevBuf(self.nref * 16)
+        return self._stage
+
+    def zero(self, buf):
+        _check(_lib.ddx_zero(self.handle, int(buf), buf.nbytes))
+
+    def sync(self):
+        _check(_lib.ddx_sync(self.handle))
+
+    def h2d(self, buf, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > buf.nbytes:
+            raise ValueError("host array larger than device buffer")
+        _check(_lib.ddx_h2d(self.handle, int(buf), arr.ctypes.data, arr.nbytes))
+
+    def d2h(self, arr, buf):
+        if not arr.flags.c_contiguous or arr.nbytes > buf.nbytes:
+            raise ValueError("host array must be contiguous and fit the buffer")
+        _check(_lib.ddx_d2h(self.handle, arr.ctypes.data, int(buf), arr.nbytes))
+
+    # --- layout -----------------------------------------------------------
+    def upload_k(self, kc, kref):
+        """Reference-layout complex array (host) -> compact device buffer."""
+        kref = np.ascontiguousarray(kref, dtype=np.complex128)
+        if kref.shape != self.kref_shape:
+            raise ValueError("k array shape %s, expected %s" % (kref.shape, self.kref_shape))
+        st = self.staging()
+        self.h2d(st, kref)
+        _check(_lib.ddx_pack(self.handle, int(st), int(kc)))
+
+    def download_k(self, kc, out=None):
+        """Compact device buffer -> reference-layout complex array (host)."""
+        if out is None:
+            out = np.empty(self.kref_shape, dtype=np.complex128)
+        st = self.staging()
+        _check(_lib.ddx_unpack(self.handle, int(kc), int(st)))
+        self.d2h(out, st)
+        return out
+
+    def upload_x(self, xd, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != self.shape:
+            raise ValueError("x array shape %s, expected %s" % (x.shape, self.shape))
+        self.h2d(xd, x)
+
+    def download_x(self, xd, out=None):
+        if out is None:
+            out = np.empty(self.shape, dtype=np.float64)
+        self.d2h(out, xd)
+        return out
+
+    # --- numerics ---------------------------------------------------------
+    def fwd(self, x, kc):
+        _check(_lib.ddx_fwd(self.handle, int(x), int(kc)))
+
+    def bwd(self, kc, x):
+        _check(_lib.ddx_bwd(self.handle, int(kc), int(x)))
+
+    def deriv(self, kc, out, dim):
+        _check(_lib.ddx_deriv(self.handle, int(kc), int(out), int(dim)))
+
+    def project(self, comps):
+        _check(_lib.ddx_project(self.handle, ptr_array([int(c) for c in comps])))
+
+    def update(self, kind, y, out, F1, F2, coeff, order, dt):
+        _check(_lib.ddx_update(self.handle, kind, int(y), int(out), int(F1),
+                               int(F2) if F2 is not None else None,
+                               float(coeff), int(order), float(dt)))
+
+    def rhs(self, params, state, deriv, forcing=None):
+        fz = None
+        if forcing is not None:
+            fz = ptr_array([int(f) if f is not None else None for f in forcing])
+        _check(_lib.ddx_rhs(self.handle, ctypes.byref(params),
+                            ptr_array([int(s) for s in state]),
+                            ptr_array([int(d) for d in deriv]), fz))
+
+    def step(self, integrator, params, state, stage, deriv1, deriv2, nsteps, dt):
+        st = ptr_array([int(s) for s in stage]) if stage is not None else None
+        _check(_lib.ddx_step(self.handle, integrator, ctypes.byref(params),
+                             ptr_array([int(s) for s in state]), st,
+                             ptr_array([int(d) for d in deriv1]),
+                             ptr_array([int(d) for d in deriv2]),
+                             int(nsteps), float(dt)))
+
+
+def make_params(physics, nu=0., viscosity_order=1, kappa=0., eta=0., g=1., alpha_t=1.,
+                beta=1., bdir=2, rho0=1.):
+    return Params(physics, int(viscosity_order), float(nu), float(kappa), float(eta),
+                  float(g), float(alpha_t), float(beta), int(bdir), float(rho0))
+
+
+_plans = {}
+
+
+def get_plan(shape, length, precision=F64):
+    """Shared plan per (shape, length, precision)."""
+    key = (tuple(int(n) for n in shape), tuple(float(x) for x in length), precision)
+    p = _plans.get(key)
+    if p is None:
+        p = Plan(shape, length, precision)
+        _plans[key] = p
+    return p

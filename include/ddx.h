@@ -1,0 +1,165 @@
+/*
+ * ddx.h -- C ABI of libddx.so, the MI355X (gfx950) pseudospectral step.
+ *
+ * Replaces, behind dedalus-1.0's own class interfaces, the numerical work of
+ *   - FourierRepresentation.fwd_np / rev_np / fwd_fftw / rev_fftw and the
+ *     rPlan FFTW wrapper       (dedalus/data_objects/representations.py:318-357,
+ *                               dedalus/utils/fftw/_fftw.pyx:246-309)
+ *   - the 2/3 dealias kernels  (dedalus/data_objects/dealias_cy_2d.pyx:13-39,
+ *                               dealias_cy_3d.pyx:13-46, representations.py:384-408)
+ *   - deriv / k2 / projection  (representations.py:419-440, physics.py:180-195,
+ *                               physics.py:407-416, physics.py:588-599)
+ *   - Physics.RHS              (physics.py:134-149, 527-586, 664-712, 770-819)
+ *   - euler / etd1 / etd2rk1 / etd2rk2
+ *                              (dedalus/time_stepping/forward_step_cy_2d.pyx:21-120,
+ *                               forward_step_cy_3d.pyx:17-124)
+ *   - RK2mid.do_advance / RK2trap.do_advance
+ *                              (dedalus/time_stepping/time_step.py:275-309, 360-392)
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative DDX_ERR_* code; the
+ *     message of the last failure on the calling thread is ddx_last_error().
+ *   - x-space arrays are float64 (or float32 for DDX_F32 plans) in the
+ *     reference layout: (y, x) in 2D, (z, y, x) in 3D, C order.
+ *   - "kref" arrays are complex128 in the reference's transposed
+ *     half-spectrum layout: (kx, ky) = (Nx/2+1, Ny) in 2D and
+ *     (ky, kz, kx) = (Ny, Nz, Nx/2+1) in 3D (representations.py:82-93).
+ *   - "kc" arrays are the library's compact k layout: only the modes kept by
+ *     the 2/3 rule, shape (Ky, Kz, Kxp) with Kz = 1 in 2D, kx fastest and
+ *     padded to Kxp (pad entries are zero).  ddx_plan_kshape() gives the
+ *     sizes; ddx_pack / ddx_unpack convert to and from kref.
+ *   - All device pointers are plain device addresses (ddx_alloc, or any
+ *     other allocation on the same device).  The library never frees
+ *     caller memory.  Calls are asynchronous on the plan's stream except
+ *     ddx_h2d / ddx_d2h / ddx_sync, which return after the stream drains.
+ *   - One host thread per plan.
+ */
+#ifndef DDX_H
+#define DDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ddx_plan ddx_plan;
+
+enum {
+  DDX_OK = 0,
+  DDX_ERR_ARG = -1,         /* bad argument (shape, pointer, enum) */
+  DDX_ERR_HIP = -2,         /* HIP runtime error */
+  DDX_ERR_UNSUPPORTED = -3, /* valid request this build does not handle */
+  DDX_ERR_NOMEM = -4,       /* device allocation failed */
+  DDX_ERR_NODEV = -5        /* no HIP device, or ddx_init not called */
+};
+
+enum { DDX_F64 = 0, DDX_F32 = 1 };
+enum { DDX_HYDRO = 0, DDX_BOUSSINESQ = 1, DDX_MHD = 2 };
+enum { DDX_RK2MID = 0, DDX_RK2TRAP = 1 };
+enum { DDX_EULER = 0, DDX_ETD1 = 1, DDX_ETD2RK1 = 2, DDX_ETD2RK2 = 3 };
+
+/* Physics parameters (RHS.parameters of physics.py:462-465, 642-646,
+ * 753-754).  Coefficients equal to 0 mean "no integrating factor"
+ * (the reference's integrating_factor = None, physics.py:510-513). */
+typedef struct ddx_params {
+  int physics;          /* DDX_HYDRO / DDX_BOUSSINESQ / DDX_MHD */
+  int viscosity_order;  /* 'viscosity_order' */
+  double nu;            /* 'nu' */
+  double kappa;         /* 'kappa' (Boussinesq) */
+  double eta;           /* 'eta' (MHD) */
+  double g, alpha_t, beta; /* Boussinesq 'g', 'alpha_t', 'beta' */
+  int bdir;             /* 'boussinesq_direction': 0 = x, 1 = y, 2 = z */
+  double rho0;          /* MHD 'rho0' */
+} ddx_params;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Select the HIP device; must precede every other call. */
+int ddx_init(int device);
+/* Message of the last failure on this thread ("" if none). */
+const char* ddx_last_error(void);
+/* ABI version (bumped on any signature change). */
+int ddx_version(void);
+
+/* New plan for one grid: ndim 2 or 3, xshape in reference order ((y, x) or
+ * (z, y, x)), box lengths in the same order, precision DDX_F64 / DDX_F32.
+ * Replaces FourierRepresentation.set_fft / create_fftw_plans
+ * (representations.py:290-316) and _setup_k (representations.py:204-233). */
+int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
+                    ddx_plan** plan);
+int ddx_plan_destroy(ddx_plan* plan);
+/* kref[3] = reference k shape, kc[3] = (Ky, Kz, Kxp), cut[3] = kept |index|
+ * per axis (x, y, z); *nkc = Ky*Kz*Kxp. */
+int ddx_plan_kshape(const ddx_plan* plan, int64_t* kref, int64_t* kc, int64_t* cut, int64_t* nkc);
+/* Wavenumbers of axis (0 = x, 1 = y, 2 = z) in compact order (n entries). */
+int ddx_plan_ktable(const ddx_plan* plan, int axis, double* out, int64_t n);
+/* Run subsequent work of this plan on an existing HIP stream (NULL = the
+ * plan's own stream). */
+int ddx_plan_set_stream(ddx_plan* plan, void* stream);
+
+/* ---- memory --------------------------------------------------------------- */
+
+int ddx_alloc(size_t bytes, void** ptr);
+int ddx_free(void* ptr);
+int ddx_h2d(ddx_plan* plan, void* dst, const void* src, size_t bytes);
+int ddx_d2h(ddx_plan* plan, void* dst, const void* src, size_t bytes);
+/* Zero device memory with a fill kernel (safe inside captured graphs). */
+int ddx_zero(ddx_plan* plan, void* dst, size_t bytes);
+int ddx_sync(ddx_plan* plan);
+
+/* ---- layout ------------------------------------------------------------- */
+
+/* kref -> kc; modes the 2/3 rule removes are dropped (the dealias of
+ * representations.py:384-397 applied on entry). */
+int ddx_pack(ddx_plan* plan, const void* kref, void* kc);
+/* kc -> kref; removed modes are written as zero. */
+int ddx_unpack(ddx_plan* plan, const void* kc, void* kref);
+
+/* ---- transforms (representations.py:335-357) ----------------------------- */
+
+/* x -> kc: r2c, divided by N = prod(xshape), dealiased (FourierRepresentation.forward). */
+int ddx_fwd(ddx_plan* plan, const void* x, void* kc);
+/* kc -> x: unnormalised c2r of the dealiased data (FourierRepresentation.backward);
+ * the kx = 0 plane contributes its Hermitian part, as irfftn does. */
+int ddx_bwd(ddx_plan* plan, const void* kc, void* x);
+
+/* ---- per-operation kernels ------------------------------------------------- */
+
+/* out = i k_dim * kc, dim 0 = x, 1 = y, 2 = z (representations.py:419-425). */
+int ddx_deriv(ddx_plan* plan, const void* kc, void* out, int dim);
+/* In-place solenoidal projection F <- F - k (k.F)/|k|^2 of ndim components
+ * (physics.py:588-599 = fields.py:169-195). */
+int ddx_project(ddx_plan* plan, void* const* comps);
+/* One ETD / Euler sweep (forward_step_cy_{2,3}d.pyx): kind DDX_EULER,
+ * DDX_ETD1, DDX_ETD2RK1, DDX_ETD2RK2.  The integrating factor coeff*|k|^(2*order)
+ * is recomputed from the wavenumbers (coeff = 0 is the reference's
+ * integrating_factor = None).  out may alias y.  F2 is ignored for EULER
+ * and ETD1.  The 2D plans use the Taylor-series phi0 of forward_step_cy_2d.pyx:54-56,
+ * 3D plans exp() as forward_step_cy_3d.pyx:54-56. */
+int ddx_update(ddx_plan* plan, int kind, const void* y, void* out, const void* F1,
+               const void* F2, double coeff, int order, double dt);
+
+/* ---- fused right-hand side and step ---------------------------------------- */
+
+/* Physics.RHS on compact state -> compact deriv.  State component order:
+ * HYDRO u[ndim]; BOUSSINESQ u[ndim], T; MHD u[ndim], B[ndim].  forcing is
+ * NULL or an array of the same length whose non-NULL entries (compact k)
+ * are added to the tendency before the projection (velocity) or after
+ * stratification (T), as physics.py:569-572 and 710-712 do. */
+int ddx_rhs(ddx_plan* plan, const ddx_params* params, const void* const* state,
+            void* const* deriv, const void* const* forcing);
+/* nsteps RK2mid / RK2trap steps of size dt on compact state in place.
+ * stage, deriv1, deriv2 are caller buffers with the state's component
+ * count (the reference integrator's data2 / deriv1 / deriv2); stage is
+ * unused by RK2trap and may be NULL there. */
+int ddx_step(ddx_plan* plan, int integrator, const ddx_params* params, void* const* state,
+             void* const* stage, void* const* deriv1, void* const* deriv2, int nsteps,
+             double dt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDX_H */

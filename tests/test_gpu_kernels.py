@@ -1,0 +1,239 @@
+"""Parity of every libddx kernel with the numpy oracle (GPU).
+
+Each test prints the case it runs before touching the device.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import kfields, random_k, random_x, rel_l2
+from oracle import numpy_path as npth
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 8), (6, 8), (16, 16), (64, 64), (48, 64), (30, 20), (128, 128), (512, 512),
+          (8, 8, 8), (4, 6, 8), (16, 16, 16), (12, 10, 16), (48, 2, 48), (32, 32, 32)]
+LENGTHS = {2: (1.0, 2.5), 3: (2 * np.pi, 1.5, 3.0)}
+FIELDS = {'hydro': ['u'], 'boussinesq': ['u', 'T'], 'mhd': ['u', 'B']}
+BDIR = {'x': 0, 'y': 1, 'z': 2}
+
+
+def hip():
+    from dedalus import _hip
+    _hip.init()
+    return _hip
+
+
+def case(*args):
+    print("case:", *args, flush=True)
+
+
+def params_struct(hp, kind, nd, p):
+    phys = {'hydro': hp.HYDRO, 'boussinesq': hp.BOUSSINESQ, 'mhd': hp.MHD}[kind]
+    return hp.make_params(phys, nu=p.get('nu', 0.), viscosity_order=p.get('viscosity_order', 1),
+                          kappa=p.get('kappa', 0.), eta=p.get('eta', 0.), g=p.get('g', 1.),
+                          alpha_t=p.get('alpha_t', 1.), beta=p.get('beta', 1.),
+                          bdir=BDIR[p.get('boussinesq_direction', 'z' if nd == 3 else 'y')],
+                          rho0=p.get('rho0', 1.))
+
+
+def upload_fields(plan, kind, kf):
+    bufs = []
+    for name in FIELDS[kind]:
+        for arr in kf[name]:
+            b = plan.alloc_k()
+            plan.upload_k(b, arr)
+            bufs.append(b)
+    return bufs
+
+
+def download_fields(plan, kind, nd, bufs):
+    out, i = {}, 0
+    for name in FIELDS[kind]:
+        nc = nd if name in ('u', 'B') else 1
+        out[name] = [plan.download_k(bufs[i + c]) for c in range(nc)]
+        i += nc
+    return out
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=str)
+def test_pack_unpack(shape):
+    hp = hip()
+    case("pack/unpack", shape)
+    L = (2 * np.pi,) * len(shape)
+    plan = hp.get_plan(shape, L)
+    comp = npth.Component(shape, L)
+    kref = np.random.default_rng(1).standard_normal(comp.kshape) * (1 + 1j)
+    b = plan.alloc_k()
+    plan.upload_k(b, kref)
+    back = plan.download_k(b)
+    expect = np.where(comp.dealias_mask(), 0, kref)
+    assert np.array_equal(back, expect)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=str)
+def test_forward(shape):
+    hp = hip()
+    L = LENGTHS[len(shape)]
+    case("forward", shape, L)
+    plan = hp.get_plan(shape, L)
+    comp = npth.Component(shape, L)
+    x = random_x(shape, 3)
+    comp['xspace'] = x
+    ref = comp['kspace']
+    xd = plan.alloc_x()
+    kc = plan.alloc_k()
+    plan.upload_x(xd, x)
+    plan.fwd(xd, kc)
+    got = plan.download_k(kc)
+    assert rel_l2(got, ref) < 1e-14
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=str)
+@pytest.mark.parametrize("hermitian", [True, False])
+def test_backward(shape, hermitian):
+    hp = hip()
+    L = LENGTHS[len(shape)]
+    case("backward", shape, L, hermitian)
+    plan = hp.get_plan(shape, L)
+    comp = npth.Component(shape, L)
+    if hermitian:
+        k = random_k(shape, 4, L)
+    else:
+        rng = np.random.default_rng(5)
+        k = rng.standard_normal(comp.kshape) + 1j * rng.standard_normal(comp.kshape)
+        k[comp.dealias_mask()] = 0
+    comp['kspace'] = k
+    ref = comp['xspace']
+    kc = plan.alloc_k()
+    xd = plan.alloc_x()
+    plan.upload_k(kc, k)
+    plan.bwd(kc, xd)
+    got = plan.download_x(xd)
+    assert rel_l2(got, ref) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (8, 6, 8)], ids=str)
+def test_deriv_and_project(shape):
+    hp = hip()
+    nd = len(shape)
+    L = LENGTHS[nd]
+    case("deriv/project", shape)
+    plan = hp.get_plan(shape, L)
+    comps = [npth.Component(shape, L) for _ in range(nd)]
+    ks = [random_k(shape, 10 + i, L) for i in range(nd)]
+    bufs = []
+    for c, k in zip(comps, ks):
+        c['kspace'] = k
+        b = plan.alloc_k()
+        plan.upload_k(b, k)
+        bufs.append(b)
+    out = plan.alloc_k()
+    for d, name in enumerate('xyz'[:nd]):
+        plan.deriv(bufs[0], out, d)
+        assert rel_l2(plan.download_k(out), comps[0].deriv(name)) < 1e-15
+    # projection against the oracle's divX / laplace_solve / deriv sequence
+    ph = npth.IncompressibleHydro(shape, L)
+    ph.aux = npth.State(shape, L, ph.aux_list)
+    st = ph.create_fields()
+    for i in range(nd):
+        st['u'][i]['kspace'] = ks[i]
+    ph.pressure_projection(None, st)
+    plan.project(bufs)
+    for i in range(nd):
+        assert rel_l2(plan.download_k(bufs[i]), st['u'][i]['kspace']) < 1e-14
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2])
+def test_update(ndim, kind, order):
+    hp = hip()
+    shape = (32, 32) if ndim == 2 else (16, 12, 16)
+    L = LENGTHS[ndim]
+    case("update", shape, kind, order)
+    plan = hp.get_plan(shape, L)
+    comp = npth.Component(shape, L)
+    y, f1, f2 = (random_k(shape, s, L) for s in (20, 21, 22))
+    coeff, dt = 0.37, 0.013
+    IF = coeff * comp.k2() ** order
+    ref = np.empty_like(y)
+    if kind == 0:
+        npth.euler(y, ref, f1, dt)
+    elif kind == 1:
+        npth.etd1(y, ref, f1, -IF, dt, ndim)
+    elif kind == 2:
+        npth.etd2rk1(y, ref, f1, f2, -IF, dt, ndim)
+    else:
+        npth.etd2rk2(y, ref, f1, f2, -IF, dt, ndim)
+    b = [plan.alloc_k() for _ in range(4)]
+    for buf, arr in zip(b, (y, f1, f2)):
+        plan.upload_k(buf, arr)
+    plan.update(kind, b[0], b[3], b[1], b[2], coeff, order, dt)
+    assert rel_l2(plan.download_k(b[3]), ref) < 1e-14
+    # in place
+    plan.update(kind, b[0], b[0], b[1], b[2], coeff, order, dt)
+    assert rel_l2(plan.download_k(b[0]), ref) < 1e-14
+
+
+RHS_CASES = [
+    ('hydro', (16, 16), {'nu': 1e-2}),
+    ('hydro', (64, 48), {'nu': 1e-3}),
+    ('hydro', (16, 12, 8), {'nu': 1e-2}),
+    ('hydro', (32, 32, 32), {}),
+    ('boussinesq', (32, 32), {'boussinesq_direction': 'y', 'g': 1.3, 'alpha_t': 0.7, 'beta': 1.1}),
+    ('boussinesq', (16, 8, 16), {'boussinesq_direction': 'z', 'kappa': 1e-3}),
+    ('mhd', (32, 32), {'rho0': 1.3}),
+    ('mhd', (16, 16, 16), {'rho0': 0.8}),
+    ('mhd', (8, 12, 16), {}),
+]
+
+
+@pytest.mark.parametrize("kind,shape,p", RHS_CASES, ids=lambda v: str(v))
+def test_rhs(kind, shape, p):
+    hp = hip()
+    nd = len(shape)
+    L = LENGTHS[nd]
+    case("rhs", kind, shape, p)
+    kf = kfields(kind, shape, 30, L)
+    ref = npth.rhs(kind, shape, kf, L, **dict(p))
+    plan = hp.get_plan(shape, L)
+    st = upload_fields(plan, kind, kf)
+    dv = [plan.alloc_k() for _ in st]
+    plan.rhs(params_struct(hp, kind, nd, p), st, dv)
+    got = download_fields(plan, kind, nd, dv)
+    for name in FIELDS[kind]:
+        for c in range(len(ref[name])):
+            assert rel_l2(got[name][c], ref[name][c]) < 1e-12, (name, c)
+
+
+STEP_CASES = [
+    ('hydro', (32, 32), {'nu': 5e-3}, 'RK2mid', 1e-2, 10),
+    ('hydro', (32, 32), {'nu': 5e-3, 'viscosity_order': 2}, 'RK2trap', 1e-2, 10),
+    ('hydro', (16, 16, 16), {'nu': 1e-2}, 'RK2mid', 1e-2, 5),
+    ('boussinesq', (32, 32), {'boussinesq_direction': 'y', 'nu': 1e-3, 'kappa': 2e-3}, 'RK2mid', 1e-2, 10),
+    ('mhd', (32, 32), {'nu': 1e-3, 'eta': 2e-3}, 'RK2mid', 1e-2, 10),
+    ('mhd', (16, 16, 16), {'nu': 1e-3, 'eta': 1e-3}, 'RK2mid', 5e-3, 5),
+    ('mhd', (16, 16, 16), {'nu': 1e-3, 'eta': 1e-3}, 'RK2trap', 5e-3, 5),
+]
+
+
+@pytest.mark.parametrize("kind,shape,p,integ,dt,nsteps", STEP_CASES, ids=lambda v: str(v))
+def test_steps(kind, shape, p, integ, dt, nsteps):
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("steps", kind, shape, p, integ, nsteps)
+    kf = kfields(kind, shape, 40, L)
+    ref = npth.run(kind, shape, kf, dt, nsteps, integ, L, **dict(p))
+    plan = hp.get_plan(shape, L)
+    st = upload_fields(plan, kind, kf)
+    stage = [plan.alloc_k() for _ in st]
+    d1 = [plan.alloc_k() for _ in st]
+    d2 = [plan.alloc_k() for _ in st]
+    integrator = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+    plan.step(integrator, params_struct(hp, kind, nd, p), st, stage, d1, d2, nsteps, dt)
+    got = download_fields(plan, kind, nd, st)
+    for name in FIELDS[kind]:
+        for c in range(len(ref[name])):
+            assert rel_l2(got[name][c], ref[name][c]) < 1e-12, (name, c)
