@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X pseudospectral step (one JSON line on stdout).
+
+A step is one RK2mid step (two right-hand sides, two ETD sweeps) of the
+workload, with the state already resident in HBM.  Default workload: 3D
+incompressible MHD, 256^3, FP64, seeded solenoidal u and B with
+E(k) ~ k^4 exp(-2 (k/4)^2), rms 1 each, nu = eta = 1e-3, dt = 2.5e-4.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Fields of the JSON line besides the driver's contract:
+  roofline      dominant kernel class (most GPU time in the timed region):
+                algorithmic bytes per launch / mean launch duration, both
+                measured with HIP events on the plan's stream inside the
+                timed region; traffic = HBM bytes per launch from the
+                rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/ when
+                one exists for this kernel and workload, else null
+  step_roofline the whole step against the byte model of B_alg per
+                point-step (compulsory traffic of the minimal dataflow)
+  cpu_baseline  the numpy oracle (a restatement of the reference's numpy
+                path) timed on this host for one step of the same workload
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dedalus-1.0_amd"))
+
+METRIC = "Mpt-steps/s at 256³ FP64 + %HBM peak; closure speedup at equal spectrum err"
+HBM_PEAK_GBS = 8000.0
+
+# compulsory bytes per point-step of the minimal dataflow (FP64), per workload
+B_ALG_PER_POINT_STEP = {
+    "mhd3d_256": 1206.0,
+    "hydro3d_128": 654.4,
+    "boussinesq2d_512": 521.3,
+    "taylor_green_64": 375.5,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def gpu_run(w, steps, warmup):
+    from dedalus import _hip
+
+    nd = len(w["shape"])
+    plan = _hip.Plan(w["shape"], w["length"], _hip.F64)
+    names = {"hydro": ["u"], "boussinesq": ["u", "T"], "mhd": ["u", "B"]}[w["kind"]]
+    state = []
+    for nm in names:
+        for arr in w["fields"][nm]:
+            b = plan.alloc_k()
+            plan.upload_k(b, arr)
+            state.append(b)
+    stage = [plan.alloc_k() for _ in state]
+    d1 = [plan.alloc_k() for _ in state]
+    d2 = [plan.alloc_k() for _ in state]
+    p = w["params"]
+    phys = {"hydro": _hip.HYDRO, "boussinesq": _hip.BOUSSINESQ, "mhd": _hip.MHD}[w["kind"]]
+    bdir = {"x": 0, "y": 1, "z": 2}[p.get("boussinesq_direction", "z" if nd == 3 else "y")]
+    prm = _hip.make_params(phys, nu=p.get("nu", 0.), viscosity_order=p.get("viscosity_order", 1),
+                           kappa=p.get("kappa", 0.), eta=p.get("eta", 0.), g=p.get("g", 1.),
+                           alpha_t=p.get("alpha_t", 1.), beta=p.get("beta", 1.), bdir=bdir,
+                           rho0=p.get("rho0", 1.))
+    integ = _hip.RK2MID if w["integrator"] == "RK2mid" else _hip.RK2TRAP
+    dt = w["dt"]
+    log("warmup %d steps" % warmup)
+    plan.step(integ, prm, state, stage, d1, d2, warmup, dt)
+    plan.sync()
+    log("timed %d steps" % steps)
+    plan.prof_enable(True)
+    plan.sync()
+    t0 = time.perf_counter()
+    plan.step(integ, prm, state, stage, d1, d2, steps, dt)
+    plan.sync()
+    t1 = time.perf_counter()
+    prof = plan.prof_read()
+    plan.prof_enable(False)
+    # sanity: finite state after the run
+    import numpy as np
+    u0 = plan.download_k(state[0])
+    if not np.all(np.isfinite(u0)):
+        raise RuntimeError("non-finite state after the timed steps")
+    return t1 - t0, prof
+
+
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch from a committed rocprofv3 summary, if present."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_pmc.txt" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch"):
+                return float(line.split("=", 1)[1])
+    return None
+
+
+def cpu_baseline(w):
+    """One RK2mid step of the same workload on the numpy oracle, one process."""
+    sys.path.insert(0, ROOT)
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import numpy as np
+    from oracle import numpy_path as npth
+
+    ph = npth.make_physics(w["kind"], w["shape"], w["length"], **dict(w["params"]))
+    data = npth.load_state(ph, w["fields"])
+    ti = (npth.RK2mid if w["integrator"] == "RK2mid" else npth.RK2trap)(ph)
+    log("cpu baseline: one oracle step")
+    t0 = time.perf_counter()
+    ti.do_advance(data, w["dt"])
+    t1 = time.perf_counter()
+    npts = float(np.prod(w["shape"]))
+    return {
+        "value": npts / (t1 - t0) / 1e6,
+        "unit": "Mpt-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "one RK2mid step of the same %s workload (%s) on the numpy oracle, "
+                  "single thread (pocketfft); %.1f s; host %s, %d logical CPUs, numpy %s"
+                  % (w["name"], "x".join(map(str, w["shape"])), t1 - t0, cpu_model(),
+                     os.cpu_count() or 0, np.__version__),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mhd3d_256")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the pseudospectral step here is built and measured "
+              "for a single MI355X (multi-GPU slab decomposition is design-only); "
+              "got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from dedalus.init_cond.seeded import workload
+    w = workload(args.workload)
+    npts = 1
+    for n in w["shape"]:
+        npts *= n
+    wall, prof = gpu_run(w, args.steps, args.warmup)
+    ms_step = wall / args.steps * 1e3
+    value = npts * args.steps / wall / 1e6
+
+    dom = max(prof, key=lambda k: prof[k][1])
+    launches, ms_tot, by_tot = prof[dom]
+    per_launch_bytes = by_tot / launches
+    per_launch_s = ms_tot / launches / 1e3
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(dom, w["name"]),
+            "alg_bytes_per_launch": per_launch_bytes,
+            "mean_launch_us": round(per_launch_s * 1e6, 2),
+            "launches": launches,
+            "gpu_time_share": round(ms_tot / sum(v[1] for v in prof.values()), 4)}
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
+                   "GBs": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    balg = B_ALG_PER_POINT_STEP.get(w["name"])
+    step_roof = None
+    if balg:
+        gbs = balg * npts * args.steps / wall / 1e9
+        step_roof = {"bytes_per_point_step": balg, "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded initial conditions (dedalus.init_cond.seeded.%s)" % w["name"],
+        "config": {"workload": w["name"], "physics": w["kind"],
+                   "grid": list(w["shape"]), "integrator": w["integrator"], "dt": w["dt"],
+                   "params": w["params"]},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -111,12 +111,61 @@ struct ddx_plan {
   void* wsA = nullptr;
   void* wsB = nullptr;
   size_t ws_bytes = 0;
+  // kernel instrumentation (HIP events around each launch when enabled)
+  double* partial = nullptr;  // reduction partials (device)
+  std::vector<double> hpartial;
+  bool prof = false;
+  struct Rec { int cat; hipEvent_t a, b; double bytes; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evfree;
 };
 
 namespace {
 
 size_t cpx_bytes(const ddx_plan* p) { return p->precision == DDX_F64 ? 16 : 8; }
 size_t real_bytes(const ddx_plan* p) { return p->precision == DDX_F64 ? 8 : 4; }
+
+// kernel classes reported by ddx_prof_read
+enum ProfCat {
+  PC_COL_BWD_Z_SPECTRAL, PC_COL_BWD_Y_SPECTRAL, PC_COL_BWD_Z, PC_COL_BWD_Y,
+  PC_COL_FWD_Y, PC_COL_FWD_Z, PC_ROW_PHYS, PC_ROW_C2R, PC_ROW_R2C,
+  PC_EPILOGUE, PC_UPDATE, PC_LAYOUT, PC_POINTWISE, PC_COPY, PC_NCAT
+};
+const char* const kCatNames[PC_NCAT] = {
+  "col_bwd_z_spectral", "col_bwd_y_spectral", "col_bwd_z", "col_bwd_y",
+  "col_fwd_y", "col_fwd_z", "row_phys", "row_c2r", "row_r2c",
+  "epilogue", "update", "layout", "pointwise", "copy"};
+
+hipEvent_t take_event(ddx_plan* p) {
+  if (!p->evfree.empty()) {
+    hipEvent_t e = p->evfree.back();
+    p->evfree.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Records a start event on construction and a stop event on destruction
+// (around one kernel launch) when instrumentation is on.
+struct ProfScope {
+  ddx_plan* p;
+  int cat;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(ddx_plan* p_, int cat_, double bytes_) : p(p_), cat(cat_), bytes(bytes_) {
+    if (p->prof) {
+      a = take_event(p);
+      if (a && hipEventRecord(a, p->stream) != hipSuccess) a = nullptr;
+    }
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = take_event(p);
+    if (b && hipEventRecord(b, p->stream) == hipSuccess) p->recs.push_back({cat, a, b, bytes});
+  }
+};
 
 KGeom kgeom(const ddx_plan* p) {
   KGeom g;
@@ -167,13 +216,19 @@ int tile_width(int n, size_t cb) {
 enum ColKind { COL_PLAIN, COL_SPECTRAL };
 
 template <typename T>
-int launch_col(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
+int launch_col(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral,
+               int cat, int nsrc = 0) {
   if (nfields < 1 || nfields > DDX_MAXF) return fail(DDX_ERR_ARG, "bad field count");
   a.W = tile_width(a.n, sizeof(cpx<T>));
   const size_t lds = (size_t)2 * a.n * a.W * sizeof(cpx<T>);
   if (lds > 65536) return fail(DDX_ERR_UNSUPPORTED, "column FFT too long for LDS");
   dim3 grid((a.ncols + a.W - 1) / a.W, nslabs, nfields);
   dim3 block(256);
+  // algorithmic bytes: every valid input element read once, output written once
+  const double per_in = (double)nslabs * a.in_rows * p->Kx;
+  const double per_out = (double)nslabs * a.out_rows * p->Kx;
+  const double bytes = ((spectral ? nsrc : nfields) * per_in + nfields * per_out) * sizeof(cpx<T>);
+  ProfScope prof(p, cat, bytes);
   if (dir > 0) {
     if (spectral) hipLaunchKernelGGL((col_fft_kernel<T, 1, true>), grid, block, lds, p->stream, a);
     else hipLaunchKernelGGL((col_fft_kernel<T, 1, false>), grid, block, lds, p->stream, a);
@@ -261,6 +316,13 @@ int launch_row(ddx_plan* p, RowArgs<T>& a) {
   const size_t lds = (size_t)2 * nseq * a.nx * sizeof(cpx<T>);
   if (lds > 65536) return fail(DDX_ERR_UNSUPPORTED, "x rows too long for LDS");
   const long nrows = (long)p->ay.n * p->az.n;
+  const double cb = sizeof(cpx<T>), rb = sizeof(T);
+  double bytes;
+  int cat;
+  if (a.mode == ROW_PHYS) { cat = PC_ROW_PHYS; bytes = nrows * (double)(a.nin + a.nout) * p->Kx * cb; }
+  else if (a.mode == ROW_C2R) { cat = PC_ROW_C2R; bytes = nrows * (double)a.nin * (p->Kx * cb + a.nx * rb); }
+  else { cat = PC_ROW_R2C; bytes = nrows * (double)a.nout * (p->Kx * cb + a.nx * rb); }
+  ProfScope prof(p, cat, bytes);
   hipLaunchKernelGGL((row_kernel<T>), dim3(nrows), dim3(256), lds, p->stream, a);
   LAUNCHCHK();
   return 0;
@@ -301,16 +363,16 @@ int fwd_impl(ddx_plan* p, const T* x, cpx<T>* kc) {
   if (p->ndim == 2) {
     c.out[0] = kc;
     c.scale = T(1.0 / (double)p->npts);
-    return launch_col(p, c, 1, 1, -1, false);
+    return launch_col(p, c, 1, 1, -1, false, PC_COL_FWD_Y);
   }
   c.out[0] = B;
-  if ((rc = launch_col(p, c, p->az.n, 1, -1, false))) return rc;
+  if ((rc = launch_col(p, c, p->az.n, 1, -1, false, PC_COL_FWD_Y))) return rc;
   col_defaults(c);
   geom_fz(p, c);
   c.in[0] = B;
   c.out[0] = kc;
   c.scale = T(1.0 / (double)p->npts);
-  return launch_col(p, c, p->Ky, 1, -1, false);
+  return launch_col(p, c, p->Ky, 1, -1, false, PC_COL_FWD_Z);
 }
 
 template <typename T>
@@ -326,18 +388,18 @@ int bwd_impl(ddx_plan* p, const cpx<T>* kc, T* x) {
     geom_bz(p, c);
     c.in[0] = kc;
     c.out[0] = A;
-    if ((rc = launch_col(p, c, p->Ky, 1, 1, false))) return rc;
+    if ((rc = launch_col(p, c, p->Ky, 1, 1, false, PC_COL_BWD_Z))) return rc;
     col_defaults(c);
     geom_by(p, c);
     c.in[0] = A;
     c.out[0] = B;
-    if ((rc = launch_col(p, c, p->az.n, 1, 1, false))) return rc;
+    if ((rc = launch_col(p, c, p->az.n, 1, 1, false, PC_COL_BWD_Y))) return rc;
   } else {
     col_defaults(c);
     geom_by(p, c);
     c.in[0] = kc;
     c.out[0] = B;
-    if ((rc = launch_col(p, c, 1, 1, 1, false))) return rc;
+    if ((rc = launch_col(p, c, 1, 1, 1, false, PC_COL_BWD_Y))) return rc;
   }
   ymix = B;
   RowArgs<T> r;
@@ -439,15 +501,15 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   if (nd == 3) {
     geom_bz(p, c);
     for (int q = 0; q < L.nq; ++q) c.out[q] = A + q * fs;
-    if ((rc = launch_col(p, c, p->Ky, L.nq, 1, true))) return rc;
+    if ((rc = launch_col(p, c, p->Ky, L.nq, 1, true, PC_COL_BWD_Z_SPECTRAL, L.nstate))) return rc;
     col_defaults(c);
     geom_by(p, c);
     for (int q = 0; q < L.nq; ++q) { c.in[q] = A + q * fs; c.out[q] = B + q * fs; }
-    if ((rc = launch_col(p, c, p->az.n, L.nq, 1, false))) return rc;
+    if ((rc = launch_col(p, c, p->az.n, L.nq, 1, false, PC_COL_BWD_Y))) return rc;
   } else {
     geom_by(p, c);
     for (int q = 0; q < L.nq; ++q) c.out[q] = B + q * fs;
-    if ((rc = launch_col(p, c, 1, L.nq, 1, true))) return rc;
+    if ((rc = launch_col(p, c, 1, L.nq, 1, true, PC_COL_BWD_Y_SPECTRAL, L.nstate))) return rc;
   }
   mixed = B;
 
@@ -470,15 +532,15 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int o = 0; o < L.np; ++o) c.in[o] = A + o * fs;
   if (nd == 3) {
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
-    if ((rc = launch_col(p, c, p->az.n, L.np, -1, false))) return rc;
+    if ((rc = launch_col(p, c, p->az.n, L.np, -1, false, PC_COL_FWD_Y))) return rc;
     col_defaults(c);
     geom_fz(p, c);
     for (int o = 0; o < L.np; ++o) { c.in[o] = B + o * fs; c.out[o] = A + o * fs; }
-    if ((rc = launch_col(p, c, p->Ky, L.np, -1, false))) return rc;
+    if ((rc = launch_col(p, c, p->Ky, L.np, -1, false, PC_COL_FWD_Z))) return rc;
     Uc = A;
   } else {
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
-    if ((rc = launch_col(p, c, 1, L.np, -1, false))) return rc;
+    if ((rc = launch_col(p, c, 1, L.np, -1, false, PC_COL_FWD_Y))) return rc;
     Uc = B;
   }
 
@@ -500,6 +562,11 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   e.galpha = T(prm->g * prm->alpha_t);
   e.beta = T(prm->beta);
   const int bs = 256;
+  int nforce = 0;
+  for (int s = 0; s < L.nstate; ++s) nforce += e.forcing[s] ? 1 : 0;
+  const double kvalid = (double)p->Ky * p->Kz * p->Kx;
+  const int extra = prm->physics == DDX_BOUSSINESQ ? 2 : 0;
+  ProfScope prof(p, PC_EPILOGUE, kvalid * (L.np + L.nstate + extra + nforce) * sizeof(cpx<T>));
   hipLaunchKernelGGL((epilogue_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
   LAUNCHCHK();
   return 0;
@@ -530,6 +597,8 @@ int update_impl(ddx_plan* p, int kind, int ncomp, const cpx<T>* const* y, cpx<T>
   u.ndim = p->ndim;
   u.dt = dt;
   const int bs = 256;
+  const double kvalid = (double)p->Ky * p->Kz * p->Kx;
+  ProfScope prof(p, PC_UPDATE, kvalid * ncomp * (kind >= DDX_ETD2RK1 ? 4 : 3) * sizeof(cpx<T>));
   hipLaunchKernelGGL((update_kernel<T>), dim3((p->nk + bs - 1) / bs, ncomp), dim3(bs), 0, p->stream, u);
   LAUNCHCHK();
   return 0;
@@ -713,6 +782,9 @@ int ddx_plan_destroy(ddx_plan* p) {
   for (void* q : p->owned) (void)hipFree(q);
   if (p->wsA) (void)hipFree(p->wsA);
   if (p->wsB) (void)hipFree(p->wsB);
+  if (p->partial) (void)hipFree(p->partial);
+  for (auto& r : p->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (hipEvent_t e : p->evfree) (void)hipEventDestroy(e);
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
   return 0;
@@ -783,6 +855,23 @@ int ddx_zero(ddx_plan* p, void* dst, size_t bytes) {
   } else {
     hipLaunchKernelGGL(zero_bytes_kernel, dim3((bytes + 255) / 256), dim3(256), 0, p->stream,
                        (uint8_t*)dst, bytes);
+  }
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_copy(ddx_plan* p, void* dst, const void* src, size_t bytes) {
+  if (check_plan(p) || !dst || !src) return fail(DDX_ERR_ARG, "null argument");
+  if (bytes == 0) return 0;
+  ProfScope prof(p, PC_COPY, 2.0 * (double)bytes);
+  if (bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0) {
+    const size_t n = bytes / 16;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(copy16_kernel, dim3(blocks), dim3(256), 0, p->stream, (const uint4*)src,
+                       (uint4*)dst, n);
+  } else {
+    hipLaunchKernelGGL(copy1_kernel, dim3((bytes + 255) / 256), dim3(256), 0, p->stream,
+                       (const uint8_t*)src, (uint8_t*)dst, bytes);
   }
   LAUNCHCHK();
   return 0;
@@ -872,6 +961,136 @@ int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* st
   void* const* st = stage ? stage : state;
   return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
                            (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2, nsteps, dt);
+}
+
+int ddx_kcombine(ddx_plan* p, void* out, int nterm, const void* const* src, const double* coef,
+                 const int* op, int accumulate) {
+  if (check_plan(p) || !out || (nterm > 0 && (!src || !coef || !op)))
+    return fail(DDX_ERR_ARG, "null argument");
+  if (nterm < 0 || nterm > DDX_MAXT) return fail(DDX_ERR_ARG, "1..4 terms");
+  KCombArgs<double> a;
+  memset(&a, 0, sizeof(a));
+  for (int t = 0; t < nterm; ++t) {
+    if (!src[t]) return fail(DDX_ERR_ARG, "null source");
+    if (op[t] < -1 || op[t] > 3 || (op[t] >= 0 && op[t] <= 2 && op[t] >= p->ndim))
+      return fail(DDX_ERR_ARG, "bad operator");
+    a.src[t] = (const cpx<double>*)src[t];
+    a.coef[t] = coef[t];
+    a.op[t] = op[t];
+  }
+  a.nterm = nterm;
+  a.accumulate = accumulate;
+  a.out = (cpx<double>*)out;
+  a.g = kgeom(p);
+  a.nk = p->nk;
+  ProfScope prof(p, PC_POINTWISE, (double)p->Ky * p->Kz * p->Kx * (nterm + 1 + accumulate) * 16.0);
+  hipLaunchKernelGGL((kcombine_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ddx_xcombine(ddx_plan* p, void* out, int nterm, const void* const* a_, const void* const* b_,
+                 const double* coef, int accumulate) {
+  if (check_plan(p) || !out || (nterm > 0 && (!a_ || !coef))) return fail(DDX_ERR_ARG, "null argument");
+  if (nterm < 0 || nterm > DDX_MAXT) return fail(DDX_ERR_ARG, "1..4 terms");
+  XCombArgs<double> a;
+  memset(&a, 0, sizeof(a));
+  int nb = 0;
+  for (int t = 0; t < nterm; ++t) {
+    if (!a_[t]) return fail(DDX_ERR_ARG, "null operand");
+    a.a[t] = (const double*)a_[t];
+    a.b[t] = b_ ? (const double*)b_[t] : nullptr;
+    nb += a.b[t] ? 1 : 0;
+    a.coef[t] = coef[t];
+  }
+  a.nterm = nterm;
+  a.accumulate = accumulate;
+  a.out = (double*)out;
+  a.n = p->npts;
+  ProfScope prof(p, PC_POINTWISE, (double)p->npts * (nterm + nb + 1 + accumulate) * 8.0);
+  hipLaunchKernelGGL((xcombine_kernel<double>), dim3((p->npts + 255) / 256), dim3(256), 0, p->stream, a);
+  LAUNCHCHK();
+  return 0;
+}
+
+static int finish_reduce(ddx_plan* p, int nblk, int op, double* out) {
+  p->hpartial.resize(nblk);
+  HIPCHK(hipMemcpyAsync(p->hpartial.data(), p->partial, nblk * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  double r = 0.0;
+  for (int i = 0; i < nblk; ++i) r = (op == 0) ? std::max(r, p->hpartial[i]) : r + p->hpartial[i];
+  *out = r;
+  return 0;
+}
+
+static int ensure_partial(ddx_plan* p) {
+  if (!p->partial && hipMalloc((void**)&p->partial, 1024 * sizeof(double)) != hipSuccess)
+    return fail(DDX_ERR_NOMEM, "reduction buffer");
+  return 0;
+}
+
+int ddx_xreduce(ddx_plan* p, const void* x, int op, double* out) {
+  if (check_plan(p) || !x || !out) return fail(DDX_ERR_ARG, "null argument");
+  if (op < 0 || op > 2) return fail(DDX_ERR_ARG, "bad reduction");
+  int rc = ensure_partial(p);
+  if (rc) return rc;
+  const int nblk = (int)std::min<long>(1024, (p->npts + 255) / 256);
+  hipLaunchKernelGGL((xreduce_kernel<double>), dim3(nblk), dim3(256), 0, p->stream, (const double*)x,
+                     p->npts, op, p->partial);
+  LAUNCHCHK();
+  return finish_reduce(p, nblk, op, out);
+}
+
+int ddx_kreduce(ddx_plan* p, const void* k0, const void* k1, int op, double* out) {
+  if (check_plan(p) || !k0 || !out || (op == 1 && !k1)) return fail(DDX_ERR_ARG, "null argument");
+  if (op < 0 || op > 2) return fail(DDX_ERR_ARG, "bad reduction");
+  int rc = ensure_partial(p);
+  if (rc) return rc;
+  const int nblk = (int)std::min<long>(1024, (p->nk + 255) / 256);
+  hipLaunchKernelGGL((kreduce_kernel<double>), dim3(nblk), dim3(256), 0, p->stream,
+                     (const cpx<double>*)k0, (const cpx<double>*)k1, kgeom(p), p->nk, op, p->partial);
+  LAUNCHCHK();
+  return finish_reduce(p, nblk, 1, out);
+}
+
+int ddx_prof_enable(ddx_plan* p, int on) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  for (auto& r : p->recs) {
+    p->evfree.push_back(r.a);
+    p->evfree.push_back(r.b);
+  }
+  p->recs.clear();
+  p->prof = on != 0;
+  return 0;
+}
+
+int ddx_prof_read(ddx_plan* p, int max_cat, int* ncat, char* names, int64_t* launches,
+                  double* ms_total, double* bytes_total) {
+  if (check_plan(p) || !ncat || !names || !launches || !ms_total || !bytes_total)
+    return fail(DDX_ERR_ARG, "null argument");
+  if (max_cat < PC_NCAT) return fail(DDX_ERR_ARG, "max_cat too small");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  for (int c = 0; c < PC_NCAT; ++c) {
+    strncpy(names + 32 * c, kCatNames[c], 31);
+    names[32 * c + 31] = 0;
+    launches[c] = 0;
+    ms_total[c] = 0.0;
+    bytes_total[c] = 0.0;
+  }
+  for (auto& r : p->recs) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    launches[r.cat] += 1;
+    ms_total[r.cat] += ms;
+    bytes_total[r.cat] += r.bytes;
+    p->evfree.push_back(r.a);
+    p->evfree.push_back(r.b);
+  }
+  p->recs.clear();
+  *ncat = PC_NCAT;
+  return 0;
 }
 
 }  // extern "C"

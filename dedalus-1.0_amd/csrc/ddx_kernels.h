@@ -503,6 +503,134 @@ __global__ void project_kernel(const ProjArgs<T> a) {
   for (int i = 0; i < a.ndim; ++i) a.c[i][idx] = a.c[i][idx] - scale(kf, kk[i] / k2);
 }
 
+// out (+)= sum_t coef_t * op_t(k) * src_t on compact k arrays.
+// op: -1 identity, 0/1/2 multiply by i k_x / i k_y / i k_z, 3 divide by |k|^2
+// with |k|^2 = 0 replaced by 1.
+#define DDX_MAXT 4
+template <typename T>
+struct KCombArgs {
+  const cpx<T>* src[DDX_MAXT];
+  double coef[DDX_MAXT];
+  int op[DDX_MAXT];
+  int nterm;
+  int accumulate;
+  cpx<T>* out;
+  KGeom g;
+  long nk;
+};
+
+template <typename T>
+__global__ void kcombine_kernel(const KCombArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nk) return;
+  int col;
+  T kx, ky, kz;
+  kvec<T>(a.g, idx, col, kx, ky, kz);
+  if (col >= a.g.Kx) { a.out[idx] = mk(T(0), T(0)); return; }
+  cpx<T> acc = a.accumulate ? a.out[idx] : mk(T(0), T(0));
+#pragma unroll
+  for (int t = 0; t < DDX_MAXT; ++t) {
+    if (t < a.nterm) {
+      cpx<T> v = a.src[t][idx];
+      const int op = a.op[t];
+      if (op == 0) v = mul_is(v, kx);
+      else if (op == 1) v = mul_is(v, ky);
+      else if (op == 2) v = mul_is(v, kz);
+      else if (op == 3) {
+        T k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 == T(0)) k2 = T(1);
+        v = mk(v.x / k2, v.y / k2);
+      }
+      acc = acc + scale(v, T(a.coef[t]));
+    }
+  }
+  a.out[idx] = acc;
+}
+
+// out (+)= sum_t coef_t * a_t * b_t on x-space arrays (b_t == NULL: factor 1)
+template <typename T>
+struct XCombArgs {
+  const T* a[DDX_MAXT];
+  const T* b[DDX_MAXT];
+  double coef[DDX_MAXT];
+  int nterm;
+  int accumulate;
+  T* out;
+  long n;
+};
+
+template <typename T>
+__global__ void xcombine_kernel(const XCombArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.n) return;
+  T acc = a.accumulate ? a.out[idx] : T(0);
+#pragma unroll
+  for (int t = 0; t < DDX_MAXT; ++t) {
+    if (t < a.nterm) {
+      T v = a.a[t][idx];
+      if (a.b[t]) v *= a.b[t][idx];
+      acc += T(a.coef[t]) * v;
+    }
+  }
+  a.out[idx] = acc;
+}
+
+// Block partial reductions; the host finishes the sum / max over blocks.
+// op 0: max of a^2, 1: sum of a, 2: sum of a^2 (x space, real)
+template <typename T>
+__global__ void xreduce_kernel(const T* a, long n, int op, double* partial) {
+  __shared__ double sh[256];
+  double acc = (op == 0) ? 0.0 : 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double v = (double)a[i];
+    if (op == 0) acc = fmax(acc, v * v);
+    else if (op == 1) acc += v;
+    else acc += v * v;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      sh[threadIdx.x] = (op == 0) ? fmax(sh[threadIdx.x], sh[threadIdx.x + s]) : sh[threadIdx.x] + sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// Hermitian-weighted sums over compact k arrays (weight 1 on kx = 0, 2 elsewhere),
+// i.e. x-space means by Parseval.  op 0: sum |src|^2; 1: sum |i k x src|^2 summed
+// over the 2D curl of a (src0, src1) pair (enstrophy-like); 2: Re(src) at weight.
+template <typename T>
+__global__ void kreduce_kernel(const cpx<T>* s0, const cpx<T>* s1, KGeom g, long nk, int op,
+                               double* partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
+    int col;
+    T kx, ky, kz;
+    kvec<T>(g, i, col, kx, ky, kz);
+    if (col >= g.Kx) continue;
+    const double w = col == 0 ? 1.0 : 2.0;
+    if (op == 0) {
+      const cpx<T> v = s0[i];
+      acc += w * ((double)v.x * v.x + (double)v.y * v.y);
+    } else if (op == 1) {
+      // w_z = i kx s1 - i ky s0
+      const cpx<T> v = mul_is(s1[i], kx) - mul_is(s0[i], ky);
+      acc += w * ((double)v.x * v.x + (double)v.y * v.y);
+    } else {
+      acc += w * (double)s0[i].x;
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
 __global__ void twiddle_kernel(cpx<double>* tw, int n) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -521,6 +649,17 @@ __global__ void zero_kernel(uint32_t* p, size_t nwords) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+__global__ void copy16_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ void copy1_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 __global__ void zero_bytes_kernel(uint8_t* p, size_t n) {

@@ -73,6 +73,7 @@ SIGNATURES = {
     "ddx_h2d": (_I, [_VP, _VP, _VP, _SZ]),
     "ddx_d2h": (_I, [_VP, _VP, _VP, _SZ]),
     "ddx_zero": (_I, [_VP, _VP, _SZ]),
+    "ddx_copy": (_I, [_VP, _VP, _VP, _SZ]),
     "ddx_sync": (_I, [_VP]),
     "ddx_pack": (_I, [_VP, _VP, _VP]),
     "ddx_unpack": (_I, [_VP, _VP, _VP]),
@@ -83,6 +84,12 @@ SIGNATURES = {
     "ddx_update": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _D, _I, _D]),
     "ddx_rhs": (_I, [_VP, ctypes.POINTER(Params), _PP, _PP, _PP]),
     "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _I, _D]),
+    "ddx_kcombine": (_I, [_VP, _VP, _I, _PP, _DP, ctypes.POINTER(_I), _I]),
+    "ddx_xcombine": (_I, [_VP, _VP, _I, _PP, _PP, _DP, _I]),
+    "ddx_xreduce": (_I, [_VP, _VP, _I, _DP]),
+    "ddx_kreduce": (_I, [_VP, _VP, _VP, _I, _DP]),
+    "ddx_prof_enable": (_I, [_VP, _I]),
+    "ddx_prof_read": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.c_char_p, _I64P, _DP, _DP]),
 }
 
 _lib = None
@@ -214,6 +221,30 @@ class Plan:
     def sync(self):
         _check(_lib.ddx_sync(self.handle))
 
+    def copy(self, dst, src, nbytes=None):
+        n = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
+        _check(_lib.ddx_copy(self.handle, int(dst), int(src), n))
+
+    # --- instrumentation --------------------------------------------------
+    def prof_enable(self, on=True):
+        _check(_lib.ddx_prof_enable(self.handle, 1 if on else 0))
+
+    def prof_read(self):
+        """{kernel class: (launches, total ms, total algorithmic bytes)}."""
+        m = 32
+        n = ctypes.c_int()
+        names = ctypes.create_string_buffer(32 * m)
+        launches = (ctypes.c_int64 * m)()
+        ms = (ctypes.c_double * m)()
+        by = (ctypes.c_double * m)()
+        _check(_lib.ddx_prof_read(self.handle, m, ctypes.byref(n), names, launches, ms, by))
+        out = {}
+        for i in range(n.value):
+            if launches[i]:
+                nm = names.raw[32 * i:32 * i + 32].split(b"\0", 1)[0].decode()
+                out[nm] = (int(launches[i]), float(ms[i]), float(by[i]))
+        return out
+
     def h2d(self, buf, arr):
         arr = np.ascontiguousarray(arr)
         if arr.nbytes > buf.nbytes:
@@ -273,6 +304,33 @@ class Plan:
         _check(_lib.ddx_update(self.handle, kind, int(y), int(out), int(F1),
                                int(F2) if F2 is not None else None,
                                float(coeff), int(order), float(dt)))
+
+    def kcombine(self, out, terms, accumulate=False):
+        """out (+)= sum coef * M(op) * src over (src, coef, op) terms (op -1, 0..2, 3)."""
+        n = len(terms)
+        src = ptr_array([int(t[0]) for t in terms])
+        coef = (ctypes.c_double * max(n, 1))(*[float(t[1]) for t in terms])
+        op = (ctypes.c_int * max(n, 1))(*[int(t[2]) for t in terms])
+        _check(_lib.ddx_kcombine(self.handle, int(out), n, src, coef, op, 1 if accumulate else 0))
+
+    def xcombine(self, out, terms, accumulate=False):
+        """out (+)= sum coef * a * b over (a, b or None, coef) terms (x space)."""
+        n = len(terms)
+        a = ptr_array([int(t[0]) for t in terms])
+        b = ptr_array([int(t[1]) if t[1] is not None else None for t in terms])
+        coef = (ctypes.c_double * max(n, 1))(*[float(t[2]) for t in terms])
+        _check(_lib.ddx_xcombine(self.handle, int(out), n, a, b, coef, 1 if accumulate else 0))
+
+    def xreduce(self, x, op):
+        r = ctypes.c_double()
+        _check(_lib.ddx_xreduce(self.handle, int(x), int(op), ctypes.byref(r)))
+        return r.value
+
+    def kreduce(self, k0, op, k1=None):
+        r = ctypes.c_double()
+        _check(_lib.ddx_kreduce(self.handle, int(k0), int(k1) if k1 is not None else None,
+                                int(op), ctypes.byref(r)))
+        return r.value
 
     def rhs(self, params, state, deriv, forcing=None):
         fz = None

@@ -107,6 +107,10 @@ int ddx_h2d(ddx_plan* plan, void* dst, const void* src, size_t bytes);
 int ddx_d2h(ddx_plan* plan, void* dst, const void* src, size_t bytes);
 /* Zero device memory with a fill kernel (safe inside captured graphs). */
 int ddx_zero(ddx_plan* plan, void* dst, size_t bytes);
+/* Device-to-device copy with a streaming kernel (16-byte vector loads when
+ * both pointers and the size are 16-byte aligned).  Backs the drop-in's
+ * buffer-preserving assignment between fields (representations.py:144-166). */
+int ddx_copy(ddx_plan* plan, void* dst, const void* src, size_t bytes);
 int ddx_sync(ddx_plan* plan);
 
 /* ---- layout ------------------------------------------------------------- */
@@ -141,6 +145,26 @@ int ddx_project(ddx_plan* plan, void* const* comps);
 int ddx_update(ddx_plan* plan, int kind, const void* y, void* out, const void* F1,
                const void* F2, double coeff, int order, double dt);
 
+/* Linear combination on compact k arrays (the k-space operators of
+ * physics.py:160-195, 266-307, 374-416 and fields.py:169-195):
+ *   out (+)= sum_t coef[t] * M_t(k) * src[t],   t < nterm <= 4,
+ * M = identity (op -1), i k_x / i k_y / i k_z (op 0/1/2), 1/|k|^2 with
+ * |k|^2 = 0 replaced by 1 (op 3).  accumulate = 0 overwrites out. */
+int ddx_kcombine(ddx_plan* plan, void* out, int nterm, const void* const* src,
+                 const double* coef, const int* op, int accumulate);
+/* Pointwise products on x-space arrays (XgradY / XcrossY / XdotY of
+ * physics.py:197-372): out (+)= sum_t coef[t] * a[t] * b[t], b[t] may be NULL. */
+int ddx_xcombine(ddx_plan* plan, void* out, int nterm, const void* const* a,
+                 const void* const* b, const double* coef, int accumulate);
+/* Reduction of an x-space array: op 0 = max(x^2) (fields.py:153-157),
+ * 1 = sum(x), 2 = sum(x^2).  Synchronous; result in *out. */
+int ddx_xreduce(ddx_plan* plan, const void* x, int op, double* out);
+/* Hermitian-weighted sum over a compact k array (weight 1 on kx = 0, 2
+ * elsewhere; volume_average.py:71-96): op 0 = sum |k0|^2, op 1 = sum
+ * |i kx k1 - i ky k0|^2 (2D vorticity of the pair), op 2 = sum Re(k0).
+ * Synchronous; result in *out. */
+int ddx_kreduce(ddx_plan* plan, const void* k0, const void* k1, int op, double* out);
+
 /* ---- fused right-hand side and step ---------------------------------------- */
 
 /* Physics.RHS on compact state -> compact deriv.  State component order:
@@ -157,6 +181,18 @@ int ddx_rhs(ddx_plan* plan, const ddx_params* params, const void* const* state,
 int ddx_step(ddx_plan* plan, int integrator, const ddx_params* params, void* const* state,
              void* const* stage, void* const* deriv1, void* const* deriv2, int nsteps,
              double dt);
+
+/* ---- instrumentation (no reference counterpart; replaces the wall-clock
+ *      Timer of dedalus/utils/timer.py:29-65 at kernel granularity) ------- */
+
+/* Turn per-launch HIP-event timing on (1) or off (0); clears records. */
+int ddx_prof_enable(ddx_plan* plan, int on);
+/* Aggregate records by kernel class since the last enable/read: up to
+ * max_cat classes, names as 32-byte NUL-terminated strings, launch counts,
+ * summed event durations (ms) and summed algorithmic bytes (every valid
+ * input element read once and output element written once). */
+int ddx_prof_read(ddx_plan* plan, int max_cat, int* ncat, char* names, int64_t* launches,
+                  double* ms_total, double* bytes_total);
 
 #ifdef __cplusplus
 }

@@ -501,13 +501,15 @@ def f_taylor(k, Z):
     Z = np.asarray(Z, dtype=np.float64)
     factj = 1.
     total = np.zeros_like(Z)
+    zp = np.ones_like(Z)
     if k > 2:
         for j in range(1, k):
             factj *= j
     for j in range(k, 15):
         if j > 1:
             factj *= j
-        total = total + Z ** (j - k) / factj
+        total += zp / factj
+        zp = zp * Z
     return total
 
 
@@ -522,13 +524,19 @@ def phi_functions(Z, ndim):
     """
     Z = np.asarray(Z, dtype=np.float64)
     small = (Z < 0.5) & (Z > -0.5)
-    with np.errstate(divide='ignore', invalid='ignore'):
-        e = np.exp(Z)
-        f1b = (e - 1.) / Z
-        f2b = (f1b - 1.) / Z
-    f0 = np.where(small, f_taylor(0, Z) if ndim == 2 else e, e)
-    f1 = np.where(small, f_taylor(1, Z), f1b)
-    f2 = np.where(small, f_taylor(2, Z), f2b)
+    big = ~small
+    f0 = np.empty_like(Z)
+    f1 = np.empty_like(Z)
+    f2 = np.empty_like(Z)
+    zs = Z[small]
+    f0[small] = f_taylor(0, zs) if ndim == 2 else np.exp(zs)
+    f1[small] = f_taylor(1, zs)
+    f2[small] = f_taylor(2, zs)
+    zb = Z[big]
+    eb = np.exp(zb)
+    f0[big] = eb
+    f1[big] = (eb - 1.) / zb
+    f2[big] = (f1[big] - 1.) / zb
     return f0, f1, f2
 
 

@@ -1,0 +1,392 @@
+"""Pins of the numpy oracle against mathematics (CPU).
+
+The reference ships no tests or golden vectors for this path, so each oracle
+function is checked here against something its formula does not restate:
+closed forms, brute-force DFTs, mpmath, linear-wave frequencies and
+invariants.  A plausible slip (dropped term, wrong sign or index,
+transposed operand) fails at least one of these.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import random_k, rel_l2
+from oracle import numpy_path as npth
+
+TWO_PI = 2 * np.pi
+
+
+# --- a1: wavenumbers -------------------------------------------------------
+
+@pytest.mark.parametrize("shape,length", [((6, 8), (1.0, 2.5)), ((8, 6, 8), (TWO_PI, 3.0, 1.5)),
+                                          ((5, 4), (TWO_PI, TWO_PI))])
+def test_k_grids_closed_form(shape, length):
+    c = npth.Component(shape, length)
+    nd = len(shape)
+    # x axis: half spectrum 0..Nx/2, +Nyquist
+    nx, lx = shape[-1], length[-1]
+    kx = c.k['x'].ravel()
+    assert np.allclose(kx, np.arange(nx // 2 + 1) * TWO_PI / lx, rtol=0, atol=1e-12)
+    assert kx[-1] > 0
+    ny, ly = shape[-2], length[-2]
+    m = np.fft.fftfreq(ny) * ny
+    if ny % 2 == 0:
+        m[ny // 2] = ny // 2
+    assert np.allclose(c.k['y'].ravel(), m * TWO_PI / ly, atol=1e-12)
+    # layout: (kx, ky) in 2D, (ky, kz, kx) in 3D
+    if nd == 2:
+        assert c.kshape == (nx // 2 + 1, ny)
+        assert c.k['x'].shape == (nx // 2 + 1, 1)
+    else:
+        assert c.kshape == (ny, shape[0], nx // 2 + 1)
+        assert c.k['y'].shape == (ny, 1, 1) and c.k['z'].shape == (1, shape[0], 1)
+
+
+# --- a2/a3: transforms --------------------------------------------------------
+
+def brute_rfft(x, length=None):
+    """Direct DFT / N of a real array, then the reference k layout."""
+    nd = x.ndim
+    N = x.size
+    grids = np.meshgrid(*[np.arange(n) for n in x.shape], indexing='ij')
+    if nd == 2:
+        ny, nx = x.shape
+        out = np.zeros((nx // 2 + 1, ny), dtype=complex)
+        for a in range(nx // 2 + 1):
+            for b in range(ny):
+                ph = np.exp(-2j * np.pi * (a * grids[1] / nx + b * grids[0] / ny))
+                out[a, b] = (x * ph).sum() / N
+    else:
+        nz, ny, nx = x.shape
+        out = np.zeros((ny, nz, nx // 2 + 1), dtype=complex)
+        for b in range(ny):
+            for cz in range(nz):
+                for a in range(nx // 2 + 1):
+                    ph = np.exp(-2j * np.pi * (a * grids[2] / nx + b * grids[1] / ny + cz * grids[0] / nz))
+                    out[b, cz, a] = (x * ph).sum() / N
+    return out
+
+
+@pytest.mark.parametrize("shape", [(6, 8), (8, 6), (4, 6, 8)])
+def test_forward_brute_force_dft(shape):
+    c = npth.Component(shape, (TWO_PI,) * len(shape))
+    x = np.random.default_rng(1).standard_normal(shape)
+    c['xspace'] = x
+    got = c['kspace']
+    ref = brute_rfft(x)
+    ref[c.dealias_mask()] = 0
+    assert rel_l2(got, ref) < 1e-14
+
+
+def test_single_mode_known_answer():
+    shape = (16, 16)
+    c = npth.Component(shape, (TWO_PI, TWO_PI))
+    y, x = np.meshgrid(np.arange(16) * TWO_PI / 16, np.arange(16) * TWO_PI / 16, indexing='ij')
+    c['xspace'] = np.sin(3 * x) * np.cos(2 * y)
+    k = c['kspace']
+    nz = np.argwhere(np.abs(k) > 1e-14)
+    assert sorted(map(tuple, nz)) == [(3, 2), (3, 14)]
+    assert np.allclose(k[3, 2], -0.25j) and np.allclose(k[3, 14], -0.25j)
+
+
+@pytest.mark.parametrize("shape", [(16, 12), (8, 6, 10)])
+def test_parseval_and_round_trip(shape):
+    c = npth.Component(shape, (TWO_PI,) * len(shape))
+    k = random_k(shape, 3)
+    c['kspace'] = k
+    x = c['xspace'].copy()
+    w = np.full(k.shape, 2.0)
+    if len(shape) == 2:
+        w[0, :] = 1.0
+    else:
+        w[..., 0] = 1.0
+    assert np.isclose((x ** 2).mean(), (w * np.abs(k) ** 2).sum(), rtol=1e-13)
+    c['xspace'] = x
+    assert rel_l2(c['kspace'], k) < 1e-14
+
+
+def test_c2r_uses_hermitian_part():
+    shape = (8, 8)
+    c = npth.Component(shape, (TWO_PI, TWO_PI))
+    rng = np.random.default_rng(4)
+    k = rng.standard_normal(c.kshape) + 1j * rng.standard_normal(c.kshape)
+    k[c.dealias_mask()] = 0
+    c['kspace'] = k
+    x1 = c['xspace'].copy()
+    h = k.copy()
+    line = k[0, :]
+    h[0, :] = 0.5 * (line + np.conj(line[(-np.arange(8)) % 8]))
+    c['kspace'] = h
+    x2 = c['xspace'].copy()
+    assert rel_l2(x1, x2) < 1e-14
+
+
+# --- a4: dealias ---------------------------------------------------------------
+
+@pytest.mark.parametrize("n,kmax", [(64, 21), (128, 42), (256, 85), (512, 170), (48, 15), (450, 149)])
+def test_dealias_cutoffs(n, kmax):
+    c = npth.Component((n, n), (TWO_PI, TWO_PI))
+    keep = ~c.dealias_mask()
+    ky = c.k['y'].ravel()
+    kept_y = np.abs(ky)[keep[0, :]]
+    assert kept_y.max() == kmax
+    # kept count over the full spectrum = prod(2 ceil(N/3 - 1) + 1) (representations.py:366)
+    kept_full = (2 * keep[1:, :].sum(axis=0).max() + 1) * keep[0, :].sum()
+    assert kept_full == np.prod(2 * np.ceil(np.array([n, n]) / 3. - 1) + 1)
+
+
+# --- a6/a7/a11: deriv, k2, projection ---------------------------------------------
+
+def test_deriv_single_mode():
+    shape = (8, 16, 12)
+    L = (TWO_PI, 2.0, 3.0)
+    c = npth.Component(shape, L)
+    k = np.zeros(c.kshape, complex)
+    k[2, 3, 1] = 1.0 + 2.0j     # ky index 2, kz index 3, kx index 1
+    c['kspace'] = k
+    kx, ky, kz = TWO_PI / 3.0 * 1, TWO_PI / 2.0 * 2, TWO_PI / TWO_PI * 3
+    assert np.isclose(c.deriv('x')[2, 3, 1], 1j * kx * (1 + 2j))
+    assert np.isclose(c.deriv('y')[2, 3, 1], 1j * ky * (1 + 2j))
+    assert np.isclose(c.deriv('z')[2, 3, 1], 1j * kz * (1 + 2j))
+    assert np.isclose(c.k2()[2, 3, 1], kx ** 2 + ky ** 2 + kz ** 2)
+    assert c.k2(no_zero=True)[0, 0, 0] == 1.0
+
+
+def project(shape, ks):
+    ph = npth.IncompressibleHydro(shape)
+    ph.aux = npth.State(shape, ph.length, ph.aux_list)
+    st = ph.create_fields()
+    for i, a in enumerate(ks):
+        st['u'][i]['kspace'] = a
+    ph.pressure_projection(None, st)
+    return [st['u'][i]['kspace'].copy() for i in range(len(ks))], st['u'][0]
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (8, 12, 10)])
+def test_projection_properties(shape):
+    nd = len(shape)
+    ks = [random_k(shape, 10 + i) for i in range(nd)]
+    p1, c = project(shape, ks)
+    names = 'xyz'
+    kdot = sum(c.k[names[i]] * p1[i] for i in range(nd))
+    assert np.abs(kdot).max() < 1e-14 * max(np.abs(a).max() for a in ks) * 50
+    p2, _ = project(shape, p1)
+    for a, b in zip(p1, p2):
+        assert rel_l2(b, a) < 1e-14
+    phi = random_k(shape, 20)
+    grad = [1j * c.k[names[i]] * phi for i in range(nd)]
+    p3, _ = project(shape, grad)
+    for a in p3:
+        assert np.abs(a).max() < 1e-13 * np.abs(phi).max() * 50
+
+
+# --- a9/a10: products -------------------------------------------------------------
+
+def test_product_equals_truncated_convolution():
+    """XgradY on band-limited fields equals the direct sum (8^2)."""
+    n = 8
+    shape = (n, n)
+    ph = npth.IncompressibleHydro(shape)
+    ph.aux = npth.State(shape, ph.length, ph.aux_list)
+    kf = {'u': [random_k(shape, 30 + i) for i in range(2)]}
+    data = npth.load_state(ph, kf)
+    out = ph.create_fields()
+    ph.XgradY(data['u'], data['u'], ph.aux['mathscalar'], ph.aux['mathvector'], out['u'])
+    got = [out['u'][i]['kspace'].copy() for i in range(2)]
+    # direct evaluation of sum_j u_j d_j u_i on the full spectrum of kept modes
+    c = npth.Component(shape, (TWO_PI, TWO_PI))
+    full = []
+    for a in kf['u']:
+        f = np.zeros((n, n), complex)      # (ky, kx) full
+        for kx in range(n // 2 + 1):
+            for ky in range(n):
+                f[ky, kx] = a[kx, ky]
+                f[(-ky) % n, (-kx) % n] = np.conj(a[kx, ky])
+        full.append(f)
+    freq = np.fft.fftfreq(n) * n
+    keep = np.abs(freq) < n / 3.
+    res = [np.zeros((n // 2 + 1, n), complex) for _ in range(2)]
+    for i in range(2):
+        for qy in range(n):
+            for qx in range(n):
+                for py in range(n):
+                    for px in range(n):
+                        if not (keep[qy] and keep[qx] and keep[py] and keep[px]):
+                            continue
+                        ky = freq[qy] + freq[py]
+                        kx = freq[qx] + freq[px]
+                        if abs(kx) >= n / 3. or abs(ky) >= n / 3. or kx < 0:
+                            continue
+                        grad = [1j * freq[px] * full[i][py, px], 1j * freq[py] * full[i][py, px]]
+                        val = full[0][qy, qx] * grad[0] + full[1][qy, qx] * grad[1]
+                        res[i][int(kx), int(ky) % n] += val
+    for i in range(2):
+        assert rel_l2(got[i], res[i]) < 1e-13
+
+
+def test_taylor_green_advection_is_a_gradient():
+    """(u.grad)u of 2D TG = grad(-(cos 2x + cos 2y)/4): projected it vanishes."""
+    from dedalus.init_cond.seeded import taylor_green_2d
+    shape = (32, 32)
+    out = npth.rhs('hydro', shape, {'u': taylor_green_2d(shape)})
+    for a in out['u']:
+        assert np.abs(a).max() < 1e-16
+
+
+# --- a12/a15: TG closed form, invariants ---------------------------------------
+
+def test_taylor_green_closed_form_200_steps():
+    from dedalus.init_cond.seeded import workload
+    w = workload('taylor_green_64')
+    out = npth.run('hydro', w['shape'], w['fields'], w['dt'], w['nsteps'], **w['params'])
+    fac = np.exp(-2 * 1e-3 * 0.2)
+    for a, b in zip(out['u'], w['fields']['u']):
+        assert rel_l2(a, b * fac) < 1e-14
+
+
+def energy(ks, shape):
+    from dedalus.init_cond.seeded import mean_square
+    return 0.5 * mean_square(ks, shape)
+
+
+def test_inviscid_energy_conservation_third_order():
+    """nu = 0: the Galerkin-truncated system conserves energy; RK2 drifts at O(dt^3)."""
+    shape = (32, 32)
+    ks = project(shape, [random_k(shape, 40 + i, slope=1.0) for i in range(2)])[0]
+    s = np.sqrt(0.5 / energy(ks, shape))
+    kf = {'u': [a * s for a in ks]}
+    e0 = energy(kf['u'], shape)
+    errs = []
+    for dt in (1e-2, 5e-3):
+        out = npth.run('hydro', shape, kf, dt, int(round(0.2 / dt)))
+        errs.append(abs(energy(out['u'], shape) - e0))
+    assert errs[0] < 1e-4 * e0
+    assert errs[1] < errs[0] / 6.0
+
+
+def test_divergence_and_hermitian_preserved():
+    from dedalus.init_cond.seeded import workload
+    w = workload('mhd3d_256', shape=(12, 12, 12))
+    out = npth.run('mhd', w['shape'], w['fields'], 1e-3, 3, **w['params'])
+    c = npth.Component(w['shape'], w['length'])
+    for name in ('u', 'B'):
+        div = sum(1j * c.k['xyz'[i]] * out[name][i] for i in range(3))
+        assert np.abs(div).max() < 1e-14
+        for a in out[name]:
+            plane = a[:, :, 0]
+            mirror = np.conj(plane[(-np.arange(12)) % 12][:, (-np.arange(12)) % 12])
+            assert np.abs(plane - mirror).max() < 1e-15
+
+
+# --- a13: Boussinesq internal gravity wave -----------------------------------------
+
+def test_gravity_wave_frequency():
+    """Linear g-mode with N = 1: w = N kx / |k| (2d_gmode_kx1_kz1.py:44-54)."""
+    shape = (16, 16)
+    ph = npth.make_physics('boussinesq', shape, boussinesq_direction='y')
+    c = npth.Component(shape, (TWO_PI, TWO_PI))
+    kx, ky = 1, 1
+    amp = 1e-7
+    omega = np.sqrt(kx ** 2 / (kx ** 2 + ky ** 2))
+    ux = np.zeros(c.kshape, complex)
+    uy = np.zeros(c.kshape, complex)
+    T = np.zeros(c.kshape, complex)
+    uy[kx, ky] = amp
+    ux[kx, ky] = -ky / kx * amp
+    T[kx, ky] = -1j / omega * amp
+    data = npth.load_state(ph, {'u': [ux, uy], 'T': [T]})
+    ti = npth.RK2mid(ph)
+    period = TWO_PI / omega
+    nsteps = 400
+    dt = period / nsteps
+    for _ in range(nsteps):
+        ti.do_advance(data, dt)
+    # after one period the mode returns to its initial value (phase error O(dt^2))
+    got = data['u']['y']['kspace'][kx, ky]
+    assert abs(got - amp) < 1e-3 * amp
+    quarter = npth.load_state(ph, {'u': [ux, uy], 'T': [T]})
+    ti2 = npth.RK2mid(ph)
+    for _ in range(nsteps // 4):
+        ti2.do_advance(quarter, dt)
+    # uy ~ exp(-i w t): a quarter period turns the phase by -pi/2
+    g = quarter['u']['y']['kspace'][kx, ky] / amp
+    assert abs(g - (-1j)) < 1e-3
+
+
+# --- a14: MHD Alfven wave ----------------------------------------------------------
+
+def test_alfven_wave_frequency():
+    """u1 along y on B0 x-hat with k along x: w = k B0 / sqrt(4 pi rho0) (alfven_wave.py:28-33)."""
+    shape = (8, 8, 8)
+    ph = npth.make_physics('mhd', shape)
+    c = npth.Component(shape, (TWO_PI,) * 3)
+    B0 = 1.0
+    vA = B0 / np.sqrt(4 * np.pi)
+    kx = 1
+    omega = kx * vA
+    amp = 1e-7
+    z = lambda: np.zeros(c.kshape, complex)
+    u = [z(), z(), z()]
+    B = [z(), z(), z()]
+    B[0][0, 0, 0] = B0
+    # u_y = amp cos(kx x - w t), B_y = -amp B0 kx/w cos(...)  -> k-space entry at (ky=0,kz=0,kx=1)
+    u[1][0, 0, kx] = amp / 2
+    B[1][0, 0, kx] = -amp * B0 * kx / omega / 2
+    data = npth.load_state(ph, {'u': u, 'B': B})
+    ti = npth.RK2mid(ph)
+    period = TWO_PI / omega
+    n = 400
+    for _ in range(n // 4):
+        ti.do_advance(data, period / n)
+    g = data['u']['y']['kspace'][0, 0, kx] / (amp / 2)
+    assert abs(g - np.exp(-1j * np.pi / 2)) < 1e-3
+    div = sum(1j * c.k['xyz'[i]] * data['B'][i]['kspace'] for i in range(3))
+    assert np.abs(div).max() < 1e-20
+
+
+# --- a17: ETD phi functions ----------------------------------------------------------
+
+ZS = [1e-8, -1e-8, 0.49, -0.49, 0.5, -0.5, -3.0, -50.0, 2.0, 0.3]
+
+
+@pytest.mark.parametrize("Z", ZS)
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_phi_functions_against_mpmath(Z, ndim):
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    z = mp.mpf(Z)
+    e = mp.exp(z)
+    p0 = e
+    p1 = (e - 1) / z
+    p2 = (e - 1 - z) / z ** 2
+    f0, f1, f2 = npth.phi_functions(np.array([Z]), ndim)
+    # Taylor branch truncates at degree 14: error ~ |Z|^15/15! < 1e-16 for |Z| < 0.5
+    assert abs(f0[0] - float(p0)) <= 4e-16 * abs(float(p0))
+    assert abs(f1[0] - float(p1)) <= 4e-16 * abs(float(p1)) + (1e-15 if abs(Z) >= 0.5 else 0)
+    tol2 = 4e-16 * abs(float(p2)) + (1e-15 / Z ** 2 if abs(Z) >= 0.5 else 0)
+    assert abs(f2[0] - float(p2)) <= tol2
+
+
+def test_phi_branches_match_reference_structure():
+    """|Z| < 0.5 strict takes the series; 2D phi0 by series, 3D by exp; Z == 0 Euler."""
+    z = np.array([0.5, -0.5, 0.4999999])
+    f0_2, f1_2, _ = npth.phi_functions(z, 2)
+    f0_3, _, _ = npth.phi_functions(z, 3)
+    assert f1_2[0] == (np.exp(0.5) - 1.) / 0.5 and f1_2[1] == (np.exp(-0.5) - 1.) / -0.5
+    assert f0_3[2] == np.exp(0.4999999)
+    assert f0_2[2] == npth.f_taylor(0, np.array([0.4999999]))[0]
+    y = np.array([1.0 + 1j])
+    out = np.empty_like(y)
+    npth.etd1(y, out, np.array([2.0 + 0j]), np.array([0.0]), 0.1, 2)
+    assert out[0] == y[0] + 0.1 * 2.0
+
+
+def test_etd_exact_for_linear_forcing():
+    """dy/dt = c y + F with constant F: ETD1 is exact, ETD2RK2 too."""
+    c, F, y0, dt = -3.0, 0.7 + 0.2j, 1.0 - 0.5j, 0.37
+    exact = y0 * np.exp(c * dt) + F * (np.exp(c * dt) - 1) / c
+    out = np.empty(1, complex)
+    npth.etd1(np.array([y0]), out, np.array([F]), np.array([c]), dt, 3)
+    assert abs(out[0] - exact) < 1e-15
+    npth.etd2rk2(np.array([y0]), out, np.array([F]), np.array([F]), np.array([c]), dt, 3)
+    assert abs(out[0] - exact) < 1e-15
