@@ -148,6 +148,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mhd3d_256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the timed region, copy this many MiB once with the library's "
+                         "copy kernel (a known-size reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the pseudospectral step here is built and measured "
@@ -161,6 +164,15 @@ def main():
     for n in w["shape"]:
         npts *= n
     wall, prof = gpu_run(w, args.steps, args.warmup)
+    if args.calibrate_copy_mib:
+        from dedalus import _hip
+        n = args.calibrate_copy_mib << 20
+        src, dst = _hip.DevBuf(n), _hip.DevBuf(n)
+        cp = _hip.Plan((8, 8), (1.0, 1.0))
+        cp.zero(src)
+        cp.copy(dst, src, n)
+        cp.sync()
+        log("calibration copy: %d bytes read, %d bytes written" % (n, n))
     ms_step = wall / args.steps * 1e3
     value = npts * args.steps / wall / 1e6
 

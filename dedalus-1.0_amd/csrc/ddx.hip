@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -14,6 +15,7 @@
 
 #include "../../include/ddx.h"
 #include "ddx_kernels.h"
+#include "ddx_fast.h"
 
 using namespace ddx;
 
@@ -114,6 +116,7 @@ struct ddx_plan {
   // kernel instrumentation (HIP events around each launch when enabled)
   double* partial = nullptr;  // reduction partials (device)
   std::vector<double> hpartial;
+  bool generic_only = false;  // DDX_FORCE_GENERIC=1: size-generic kernels everywhere
   bool prof = false;
   struct Rec { int cat; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
@@ -215,6 +218,34 @@ int tile_width(int n, size_t cb) {
 
 enum ColKind { COL_PLAIN, COL_SPECTRAL };
 
+// Size-specialised column kernel for n in {64, 128, 256, 512}; false if none.
+template <typename T, int N>
+void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
+  using PL = ColPlan<N>;
+  dim3 grid((a.ncols + PL::W - 1) / PL::W, nslabs, nfields);
+  dim3 block(PL::LANES * PL::W);
+  a.nslabs = nslabs;
+  a.nfields = nfields;
+  if (dir > 0) {
+    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true>), grid, block, 0, p->stream, a);
+    else hipLaunchKernelGGL((col_fft_fast<T, 1, N, false>), grid, block, 0, p->stream, a);
+  } else {
+    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, -1, N, true>), grid, block, 0, p->stream, a);
+    else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false>), grid, block, 0, p->stream, a);
+  }
+}
+
+template <typename T>
+bool launch_col_fast(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
+  switch (a.n) {
+    case 64: col_fast_n<T, 64>(p, a, nslabs, nfields, dir, spectral); return true;
+    case 128: col_fast_n<T, 128>(p, a, nslabs, nfields, dir, spectral); return true;
+    case 256: col_fast_n<T, 256>(p, a, nslabs, nfields, dir, spectral); return true;
+    case 512: col_fast_n<T, 512>(p, a, nslabs, nfields, dir, spectral); return true;
+    default: return false;
+  }
+}
+
 template <typename T>
 int launch_col(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral,
                int cat, int nsrc = 0) {
@@ -229,6 +260,10 @@ int launch_col(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, boo
   const double per_out = (double)nslabs * a.out_rows * p->Kx;
   const double bytes = ((spectral ? nsrc : nfields) * per_in + nfields * per_out) * sizeof(cpx<T>);
   ProfScope prof(p, cat, bytes);
+  if (!p->generic_only && launch_col_fast<T>(p, a, nslabs, nfields, dir, spectral)) {
+    LAUNCHCHK();
+    return 0;
+  }
   if (dir > 0) {
     if (spectral) hipLaunchKernelGGL((col_fft_kernel<T, 1, true>), grid, block, lds, p->stream, a);
     else hipLaunchKernelGGL((col_fft_kernel<T, 1, false>), grid, block, lds, p->stream, a);
@@ -310,6 +345,39 @@ void geom_fz(const ddx_plan* p, ColArgs<T>& a) {
   a.out_rs = p->Kxp; a.out_ss = (long)p->Kz * p->Kxp;
 }
 
+template <typename T, int N, int PH, int ND>
+void row_fast_go(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  constexpr int RPB = 64 / RowPlan<N>::LANES;
+  hipLaunchKernelGGL((row_phys_fast<T, N, PH, ND>), dim3((nrows + RPB - 1) / RPB), dim3(64), 0,
+                     p->stream, a, nrows);
+}
+
+template <typename T, int N>
+bool row_fast_n(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  const int key = a.physics * 10 + a.ndim;
+  switch (key) {
+    case PH_HYDRO * 10 + 2: row_fast_go<T, N, PH_HYDRO, 2>(p, a, nrows); return true;
+    case PH_HYDRO * 10 + 3: row_fast_go<T, N, PH_HYDRO, 3>(p, a, nrows); return true;
+    case PH_BOUSS * 10 + 2: row_fast_go<T, N, PH_BOUSS, 2>(p, a, nrows); return true;
+    case PH_BOUSS * 10 + 3: row_fast_go<T, N, PH_BOUSS, 3>(p, a, nrows); return true;
+    case PH_MHD * 10 + 2: row_fast_go<T, N, PH_MHD, 2>(p, a, nrows); return true;
+    case PH_MHD * 10 + 3: row_fast_go<T, N, PH_MHD, 3>(p, a, nrows); return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+bool launch_row_fast(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  if (a.kx > a.nx / 3 + 1 || a.nin > 12 || a.nout > 6) return false;  // LDS staging bound
+  switch (a.nx) {
+    case 64: return row_fast_n<T, 64>(p, a, nrows);
+    case 128: return row_fast_n<T, 128>(p, a, nrows);
+    case 256: return row_fast_n<T, 256>(p, a, nrows);
+    case 512: return row_fast_n<T, 512>(p, a, nrows);
+    default: return false;
+  }
+}
+
 template <typename T>
 int launch_row(ddx_plan* p, RowArgs<T>& a) {
   const int nseq = std::max((a.nin + 1) / 2, (a.nout + 1) / 2);
@@ -323,6 +391,10 @@ int launch_row(ddx_plan* p, RowArgs<T>& a) {
   else if (a.mode == ROW_C2R) { cat = PC_ROW_C2R; bytes = nrows * (double)a.nin * (p->Kx * cb + a.nx * rb); }
   else { cat = PC_ROW_R2C; bytes = nrows * (double)a.nout * (p->Kx * cb + a.nx * rb); }
   ProfScope prof(p, cat, bytes);
+  if (a.mode == ROW_PHYS && !p->generic_only && launch_row_fast<T>(p, a, nrows)) {
+    LAUNCHCHK();
+    return 0;
+  }
   hipLaunchKernelGGL((row_kernel<T>), dim3(nrows), dim3(256), lds, p->stream, a);
   LAUNCHCHK();
   return 0;
@@ -705,6 +777,10 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   if (nx % 2) return fail(DDX_ERR_UNSUPPORTED, "x extent must be even");
   ddx_plan* p = new ddx_plan();
   p->device = g_device;
+  {
+    const char* env = getenv("DDX_FORCE_GENERIC");
+    p->generic_only = env && env[0] == '1';
+  }
   p->ndim = ndim;
   p->precision = precision;
   std::string why;

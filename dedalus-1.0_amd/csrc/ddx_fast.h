@@ -1,0 +1,367 @@
+// Size-specialised FFT passes for the hot grid extents (64, 128, 256, 512).
+//
+// Each transform of length N is spread over LANES threads holding E = N/LANES
+// complex values in registers.  A Stockham pass of radix R gives each thread
+// E/R butterflies b = lane + LANES*q (q < E/R): inputs at b + r*N/R, outputs
+// at j + r*P with P the product of the earlier radices and j = (b - b%P)*R + b%P.
+// Consecutive passes exchange through LDS; the first pass loads straight
+// from global memory and the last pass stores straight to it.  The last
+// pass of a transform leaves thread `lane` holding positions b + r*N/R_last,
+// which are exactly the inputs of a first pass of radix R_last: an inverse
+// transform followed by a forward transform with the radix order reversed
+// keeps the data in registers in between (used by the x-row kernel).
+#pragma once
+
+#include "ddx_kernels.h"
+
+namespace ddx {
+
+// --- plans -------------------------------------------------------------------
+
+template <int N> struct RowPlan;          // x rows: one transform per row team
+template <> struct RowPlan<64>  { static constexpr int LANES = 16, E = 4, NP = 3; static constexpr int R[3] = {4, 4, 4}; };
+template <> struct RowPlan<128> { static constexpr int LANES = 32, E = 4, NP = 4; static constexpr int R[4] = {4, 4, 4, 2}; };
+template <> struct RowPlan<256> { static constexpr int LANES = 64, E = 4, NP = 4; static constexpr int R[4] = {4, 4, 4, 4}; };
+template <> struct RowPlan<512> { static constexpr int LANES = 64, E = 8, NP = 3; static constexpr int R[3] = {8, 8, 8}; };
+
+template <int N> struct ColPlan;          // columns: W columns per workgroup
+template <> struct ColPlan<64>  { static constexpr int LANES = 8,  E = 8,  W = 16, NP = 2; static constexpr int R[2] = {8, 8}; };
+template <> struct ColPlan<128> { static constexpr int LANES = 8,  E = 16, W = 16, NP = 2; static constexpr int R[2] = {16, 8}; };
+template <> struct ColPlan<256> { static constexpr int LANES = 16, E = 16, W = 8,  NP = 2; static constexpr int R[2] = {16, 16}; };
+template <> struct ColPlan<512> { static constexpr int LANES = 64, E = 8,  W = 4,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
+
+// LDS index of position m in the exchange written by a pass of radix R after
+// P points: runs of P contiguous outputs start every P*R positions; shifting
+// run t by t*P (R + 1 odd) spreads the runs of consecutive threads over
+// distinct banks, and consecutive positions stay nearly contiguous.
+template <int P, int R>
+__device__ __forceinline__ int xpad(int m) { return m + (m / (P * R)) * P; }
+
+// LDS entries per transform needed by the padded exchanges of a plan
+template <int N, class PL>
+constexpr int exch_size() {
+  int mx = 0;
+  for (int i = 0; i < PL::NP; ++i) mx = (N / PL::R[i]) > mx ? (N / PL::R[i]) : mx;
+  return N + mx + 1;
+}
+
+// Butterflies of one pass with twiddles from a per-thread register cache
+// (c[q*R + r] = forward twiddle of butterfly q, leg r; conjugated for D > 0).
+template <typename T, int D, int E, int R, int P>
+__device__ __forceinline__ void reg_pass_cached(cpx<T> (&v)[E], const cpx<T> (&c)[E]) {
+#pragma unroll
+  for (int q = 0; q < E / R; ++q) {
+    if (P > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cpx<T> w = c[q * R + r];
+        v[q * R + r] = v[q * R + r] * mk(w.x, D < 0 ? w.y : -w.y);
+      }
+    }
+    dft<D, R>(&v[q * R]);
+  }
+}
+
+// Butterflies of one pass on register data (twiddle + DFT_R).
+template <typename T, int D, int N, int LANES, int E, int R, int P>
+__device__ __forceinline__ void reg_pass(cpx<T> (&v)[E], int lane, const cpx<T>* __restrict__ tw) {
+#pragma unroll
+  for (int q = 0; q < E / R; ++q) {
+    const int b = lane + LANES * q;
+    if (P > 1) {
+      const int k = b % P;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cpx<T> w = tw[r * k * (N / (P * R))];
+        v[q * R + r] = v[q * R + r] * mk(w.x, D < 0 ? w.y : -w.y);
+      }
+    }
+    dft<D, R>(&v[q * R]);
+  }
+}
+
+template <int N, int LANES, int R, int P>
+__device__ __forceinline__ int out_pos(int lane, int q, int r) {
+  const int b = lane + LANES * q;
+  const int k = b % P;
+  return (b - k) * R + k + r * P;
+}
+
+template <int N, int LANES, int R>
+__device__ __forceinline__ int in_pos(int lane, int q, int r) {
+  return lane + LANES * q + r * (N / R);
+}
+
+// Passes of a transform whose first-pass inputs are in v.  Exchanges go
+// through buf at index xpad(pos) * WS + c (WS transforms interleaved).  PL
+// is the plan; REV runs its radices in reverse order (forward transforms
+// that follow an inverse one in the x-row kernel).
+template <typename T, int D, int N, class PL, bool REV, int WS, int I = 0, int P = 1>
+__device__ __forceinline__ void run_passes(cpx<T> (&v)[PL::E], int lane, int c, cpx<T>* buf,
+                                           const cpx<T>* __restrict__ tw) {
+  constexpr int NP = PL::NP;
+  constexpr int R = PL::R[REV ? NP - 1 - I : I];
+  reg_pass<T, D, N, PL::LANES, PL::E, R, P>(v, lane, tw);
+  if constexpr (I + 1 < NP) {
+    constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
+#pragma unroll
+    for (int q = 0; q < PL::E / R; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        buf[xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, r)) * WS + c] = v[q * R + r];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PL::E / R2; ++q)
+#pragma unroll
+      for (int r = 0; r < R2; ++r)
+        v[q * R2 + r] = buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c];
+    __syncthreads();
+    run_passes<T, D, N, PL, REV, WS, I + 1, P * R>(v, lane, c, buf, tw);
+  }
+}
+
+// Register twiddle cache of a plan: pass I of the (REV-ordered) radix
+// sequence gets E entries, c[I][q*R + r] = tw[r * (b % P) * N/(P R)].
+template <typename T, int N, class PL, bool REV, int I = 0, int P = 1>
+__device__ __forceinline__ void fill_twiddles(cpx<T> (&c)[PL::NP][PL::E], int lane,
+                                              const cpx<T>* __restrict__ tw) {
+  constexpr int NP = PL::NP;
+  constexpr int R = PL::R[REV ? NP - 1 - I : I];
+#pragma unroll
+  for (int q = 0; q < PL::E / R; ++q) {
+    const int k = (lane + PL::LANES * q) % P;
+#pragma unroll
+    for (int r = 0; r < R; ++r) c[I][q * R + r] = (P > 1 && r > 0) ? tw[r * k * (N / (P * R))] : mk(T(1), T(0));
+  }
+  if constexpr (I + 1 < NP) fill_twiddles<T, N, PL, REV, I + 1, P * R>(c, lane, tw);
+}
+
+// run_passes with twiddles from a register cache filled by fill_twiddles
+template <typename T, int D, int N, class PL, bool REV, int WS, int I = 0, int P = 1>
+__device__ __forceinline__ void run_passes_cached(cpx<T> (&v)[PL::E], int lane, int c, cpx<T>* buf,
+                                                  const cpx<T> (&twc)[PL::NP][PL::E]) {
+  constexpr int NP = PL::NP;
+  constexpr int R = PL::R[REV ? NP - 1 - I : I];
+  reg_pass_cached<T, D, PL::E, R, P>(v, twc[I]);
+  if constexpr (I + 1 < NP) {
+    constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
+#pragma unroll
+    for (int q = 0; q < PL::E / R; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        buf[xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, r)) * WS + c] = v[q * R + r];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PL::E / R2; ++q)
+#pragma unroll
+      for (int r = 0; r < R2; ++r)
+        v[q * R2 + r] = buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c];
+    __syncthreads();
+    run_passes_cached<T, D, N, PL, REV, WS, I + 1, P * R>(v, lane, c, buf, twc);
+  }
+}
+
+template <class PL>
+constexpr bool uniform_radix() {
+  for (int i = 1; i < PL::NP; ++i)
+    if (PL::R[i] != PL::R[0]) return false;
+  return true;
+}
+
+// positions held after the last pass (R_last, P_last = N / R_last): b + r*N/R_last
+template <int N, class PL, bool REV>
+__device__ __forceinline__ int final_pos(int lane, int q, int r) {
+  constexpr int RL = PL::R[REV ? 0 : PL::NP - 1];
+  return lane + PL::LANES * q + r * (N / RL);
+}
+
+// ---------------------------------------------------------------------------
+// x-row kernel: NIN half-spectrum rows -> x-space products -> NOUT rows
+// ---------------------------------------------------------------------------
+
+template <int PH, int ND> struct PhysIO;
+template <> struct PhysIO<PH_HYDRO, 3> { static constexpr int NIN = 6, NOUT = 3; };
+template <> struct PhysIO<PH_HYDRO, 2> { static constexpr int NIN = 3, NOUT = 2; };
+template <> struct PhysIO<PH_BOUSS, 3> { static constexpr int NIN = 10, NOUT = 4; };
+template <> struct PhysIO<PH_BOUSS, 2> { static constexpr int NIN = 6, NOUT = 3; };
+template <> struct PhysIO<PH_MHD, 3>   { static constexpr int NIN = 12, NOUT = 6; };
+template <> struct PhysIO<PH_MHD, 2>   { static constexpr int NIN = 6, NOUT = 3; };
+
+template <typename T, int N, int PH, int ND>
+__global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nrows) {
+  using PL = RowPlan<N>;
+  constexpr int LANES = PL::LANES, E = PL::E;
+  constexpr int RPB = 64 / LANES;
+  constexpr int NIN = PhysIO<PH, ND>::NIN, NOUT = PhysIO<PH, ND>::NOUT;
+  constexpr int NPI = (NIN + 1) / 2, NPO = (NOUT + 1) / 2;
+  constexpr int R0 = PL::R[0];
+  constexpr int BUF = exch_size<N, PL>();
+  __shared__ cpx<T> lds[RPB][BUF];
+  const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  const long r = (long)blockIdx.x * RPB + team;
+  const bool live = r < nrows;
+  cpx<T>* buf = lds[team];
+  const int cx = a.kx - 1;
+  const cpx<T>* __restrict__ tw = a.tw;
+  constexpr bool UNI = uniform_radix<PL>();
+  cpx<T> twi[PL::NP][E];
+  fill_twiddles<T, N, PL, false>(twi, lane, tw);
+
+  T q[NIN][E];
+#pragma unroll
+  for (int p = 0; p < NPI; ++p) {
+    cpx<T> v[E];
+    const bool hasb = (2 * p + 1 < NIN);
+    const cpx<T>* __restrict__ ia = a.in[2 * p] + r * a.kxp;
+    const cpx<T>* __restrict__ ib = a.in[hasb ? 2 * p + 1 : 2 * p] + r * a.kxp;
+#pragma unroll
+    for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+      for (int rr = 0; rr < R0; ++rr) {
+        const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+        cpx<T> va = mk(T(0), T(0)), vb = mk(T(0), T(0));
+        if (!live) {
+        } else if (m <= cx) {
+          va = ia[m];
+          if (hasb) vb = ib[m];
+          if (m == 0) { va.y = T(0); vb.y = T(0); }
+        } else if (m >= N - cx) {
+          va = conj(ia[N - m]);
+          if (hasb) vb = conj(ib[N - m]);
+        }
+        v[qq * R0 + rr] = mk(va.x - vb.y, va.y + vb.x);
+      }
+    run_passes_cached<T, 1, N, PL, false, 1>(v, lane, 0, buf, twi);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      q[2 * p][e] = v[e].x;
+      if (hasb) q[2 * p + 1][e] = v[e].y;
+    }
+    __syncthreads();
+  }
+
+  T o[NOUT][E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    T qi[12], oi[6];
+#pragma unroll
+    for (int f = 0; f < 12; ++f) qi[f] = f < NIN ? q[f][e] : T(0);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) oi[f] = T(0);
+    row_physics<T>(PH, ND, a.fpr_inv, qi, oi);
+#pragma unroll
+    for (int f = 0; f < NOUT; ++f) o[f][e] = oi[f];
+  }
+
+  cpx<T> twf[PL::NP][E];
+  if constexpr (UNI) {
+#pragma unroll
+    for (int i = 0; i < PL::NP; ++i)
+#pragma unroll
+      for (int e = 0; e < E; ++e) twf[i][e] = twi[i][e];
+  } else {
+    fill_twiddles<T, N, PL, true>(twf, lane, tw);
+  }
+#pragma unroll
+  for (int p = 0; p < NPO; ++p) {
+    cpx<T> v[E];
+    const bool hasb = (2 * p + 1 < NOUT);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = mk(o[2 * p][e], hasb ? o[2 * p + 1][e] : T(0));
+    run_passes_cached<T, -1, N, PL, true, 1>(v, lane, 0, buf, twf);
+    constexpr int RL = PL::R[0];
+#pragma unroll
+    for (int qq = 0; qq < E / RL; ++qq)
+#pragma unroll
+      for (int rr = 0; rr < RL; ++rr) buf[final_pos<N, PL, true>(lane, qq, rr)] = v[qq * RL + rr];
+    __syncthreads();
+    if (live) {
+      cpx<T>* __restrict__ oa = a.out[2 * p] + r * a.kxp;
+      cpx<T>* __restrict__ ob = hasb ? a.out[2 * p + 1] + r * a.kxp : nullptr;
+      for (int k = lane; k < a.kxp; k += LANES) {
+        cpx<T> ra = mk(T(0), T(0)), rb = mk(T(0), T(0));
+        if (k < a.kx) {
+          const cpx<T> zk = buf[k];
+          const cpx<T> zm = conj(buf[(N - k) % N]);
+          const cpx<T> s = zk + zm, d = zk - zm;
+          ra = mk(T(0.5) * s.x, T(0.5) * s.y);
+          rb = mk(T(0.5) * d.y, T(-0.5) * d.x);
+        }
+        oa[k] = ra;
+        if (hasb) ob[k] = rb;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column kernel: W columns per workgroup, LANES threads per column
+// ---------------------------------------------------------------------------
+
+template <typename T, int D, int N, bool SPECTRAL>
+__global__ void __launch_bounds__(ColPlan<N>::LANES * ColPlan<N>::W)
+col_fft_fast(const ColArgs<T> a) {
+  using PL = ColPlan<N>;
+  constexpr int LANES = PL::LANES, E = PL::E, W = PL::W;
+  constexpr int R0 = PL::R[0];
+  constexpr int RL = PL::R[PL::NP - 1];
+  constexpr int BUF = exch_size<N, PL>();
+  __shared__ cpx<T> lds[BUF * W];
+  const int c = threadIdx.x % W, lane = threadIdx.x / W;
+  const int f = blockIdx.z, tile = blockIdx.x;
+  const long slab = blockIdx.y;
+  const int col = tile * W + c;
+  const bool live = col < a.ncols;
+  cpx<T> v[E];
+#pragma unroll
+  for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < R0; ++rr) {
+      const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+      cpx<T> x = mk(T(0), T(0));
+      int row;
+      if (a.in_c < 0) row = m;
+      else if (m <= a.in_c) row = m;
+      else if (m >= N - a.in_c) row = m - (N - a.in_rows);
+      else row = -1;
+      if (live && row >= 0) {
+        const long off = slab * a.in_ss + (long)row * a.in_rs + col;
+        if (SPECTRAL) {
+          const int nterm = a.rec[f].nterm;
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            if (t < nterm) {
+              const Term tm = a.rec[f].t[t];
+              cpx<T> s = a.src[tm.src][off];
+              if (tm.dim >= 0) {
+                const double kk = (tm.dim == 0) ? a.k_col[col]
+                                  : (tm.dim == a.row_dim ? a.k_row[row] : a.k_slab[slab]);
+                s = mul_is(s, T(kk));
+              }
+              x = x + scale(s, T(tm.coef));
+            }
+          }
+        } else {
+          x = a.in[f][off];
+        }
+      }
+      v[qq * R0 + rr] = x;
+    }
+  run_passes<T, D, N, PL, false, W>(v, lane, c, lds, a.tw);
+  if (!live) return;
+#pragma unroll
+  for (int qq = 0; qq < E / RL; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < RL; ++rr) {
+      const int m = final_pos<N, PL, false>(lane, qq, rr);
+      int j;
+      if (a.out_c < 0) j = m;
+      else if (m <= a.out_c) j = m;
+      else if (m >= N - a.out_c) j = m - (N - a.out_rows);
+      else j = -1;
+      if (j >= 0) a.out[f][slab * a.out_ss + (long)j * a.out_rs + col] = scale(v[qq * RL + rr], a.scale);
+    }
+}
+
+}  // namespace ddx

@@ -49,6 +49,7 @@ struct ColArgs {
   int ncols;             // columns per slab
   int W;                 // tile width (columns per workgroup)
   int row_dim, slab_dim; // physical axis (0 x, 1 y, 2 z) of rows / slabs, -1 none
+  int nslabs, nfields;   // grid extents (used by the remapped 1D grids)
   T scale;
 };
 
