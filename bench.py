@@ -58,11 +58,11 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def gpu_run(w, steps, warmup):
+def gpu_run(w, steps, warmup, precision="f64"):
     from dedalus import _hip
 
     nd = len(w["shape"])
-    plan = _hip.Plan(w["shape"], w["length"], _hip.F64)
+    plan = _hip.Plan(w["shape"], w["length"], _hip.F64 if precision == "f64" else _hip.F32)
     names = {"hydro": ["u"], "boussinesq": ["u", "T"], "mhd": ["u", "B"]}[w["kind"]]
     state = []
     for nm in names:
@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mhd3d_256")
+    ap.add_argument("--precision", choices=["f64", "f32"], default="f64",
+                    help="f32: FP32 transforms and products (FP64 twiddles, projection, ETD)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed region, copy this many MiB once with the library's "
@@ -163,7 +165,7 @@ def main():
     npts = 1
     for n in w["shape"]:
         npts *= n
-    wall, prof = gpu_run(w, args.steps, args.warmup)
+    wall, prof = gpu_run(w, args.steps, args.warmup, args.precision)
     if args.calibrate_copy_mib:
         from dedalus import _hip
         n = args.calibrate_copy_mib << 20
@@ -192,6 +194,8 @@ def main():
                    "GBs": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     balg = B_ALG_PER_POINT_STEP.get(w["name"])
+    if balg and args.precision == "f32":
+        balg = balg / 2.0
     step_roof = None
     if balg:
         gbs = balg * npts * args.steps / wall / 1e9
@@ -209,7 +213,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": args.precision,
         "data": "synthetic: seeded initial conditions (dedalus.init_cond.seeded.%s)" % w["name"],
         "config": {"workload": w["name"], "physics": w["kind"],
                    "grid": list(w["shape"]), "integrator": w["integrator"], "dt": w["dt"],

@@ -630,9 +630,9 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   e.physics = prm->physics;
   e.ndim = nd;
   e.bdir = prm->bdir;
-  e.s = T(1.0 / (double)p->npts);
-  e.galpha = T(prm->g * prm->alpha_t);
-  e.beta = T(prm->beta);
+  e.npts = (double)p->npts;
+  e.galpha = prm->g * prm->alpha_t;
+  e.beta = prm->beta;
   const int bs = 256;
   int nforce = 0;
   for (int s = 0; s < L.nstate; ++s) nforce += e.forcing[s] ? 1 : 0;
@@ -768,7 +768,6 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   if (g_device < 0) return fail(DDX_ERR_NODEV, "ddx_init has not been called");
   if (ndim != 2 && ndim != 3) return fail(DDX_ERR_ARG, "Must use either 2 or 3 dimensions.");
   if (precision != DDX_F64 && precision != DDX_F32) return fail(DDX_ERR_ARG, "bad precision");
-  if (precision == DDX_F32) return fail(DDX_ERR_UNSUPPORTED, "FP32 plans are not built yet");
   for (int i = 0; i < ndim; ++i) {
     if (xshape[i] < 1 || xshape[i] > (1 << 20)) return fail(DDX_ERR_ARG, "bad grid extent");
     if (!(length[i] > 0.0)) return fail(DDX_ERR_ARG, "box lengths must be positive");
@@ -829,20 +828,32 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   }
   HIPCHK(hipStreamCreate(&p->own_stream));
   p->stream = p->own_stream;
-  // FP64 twiddles generated on the device
+  // FP64 twiddles generated on the device (sincospi); FP32 plans round them
   const int ns[3] = {p->ax.n, p->ay.n, p->az.n};
   for (int i = 0; i < 3; ++i) {
     const int n = ns[i];
     if (p->tw.count(n)) continue;
-    void* t = nullptr;
-    if (hipMalloc(&t, (size_t)n * 16) != hipSuccess) {
+    void* t64 = nullptr;
+    if (hipMalloc(&t64, (size_t)n * 16) != hipSuccess) {
       ddx_plan_destroy(p);
       return fail(DDX_ERR_NOMEM, "twiddle allocation failed");
     }
-    p->owned.push_back(t);
+    p->owned.push_back(t64);
     hipLaunchKernelGGL(twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, p->stream,
-                       (cpx<double>*)t, n);
-    p->tw[n] = t;
+                       (cpx<double>*)t64, n);
+    if (precision == DDX_F64) {
+      p->tw[n] = t64;
+    } else {
+      void* t32 = nullptr;
+      if (hipMalloc(&t32, (size_t)n * 8) != hipSuccess) {
+        ddx_plan_destroy(p);
+        return fail(DDX_ERR_NOMEM, "twiddle allocation failed");
+      }
+      p->owned.push_back(t32);
+      hipLaunchKernelGGL(twiddle_f32_kernel, dim3((n + 255) / 256), dim3(256), 0, p->stream,
+                         (const cpx<double>*)t64, (cpx<float>*)t32, n);
+      p->tw[n] = t32;
+    }
   }
   if (hipStreamSynchronize(p->stream) != hipSuccess) {
     ddx_plan_destroy(p);
@@ -959,74 +970,116 @@ int ddx_sync(ddx_plan* p) {
   return 0;
 }
 
-int ddx_pack(ddx_plan* p, const void* kref, void* kc) {
-  if (check_plan(p) || !kref || !kc) return fail(DDX_ERR_ARG, "null argument");
+#define DDX_DISPATCH(p, fn, ...) \
+  ((p)->precision == DDX_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C++" {
+template <typename T>
+static int pack_t(ddx_plan* p, const void* kref, void* kc) {
   LayoutArgs L = layout_args(p);
-  hipLaunchKernelGGL((pack_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, L,
-                     (const cpx<double>*)kref, (cpx<double>*)kc);
+  ProfScope prof(p, PC_LAYOUT, (double)p->nk * (16 + sizeof(cpx<T>)));
+  hipLaunchKernelGGL((pack_kernel<T>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, L,
+                     (const cpx<double>*)kref, (cpx<T>*)kc);
   LAUNCHCHK();
   return 0;
 }
+}  // extern "C++"
+
+int ddx_pack(ddx_plan* p, const void* kref, void* kc) {
+  if (check_plan(p) || !kref || !kc) return fail(DDX_ERR_ARG, "null argument");
+  return DDX_DISPATCH(p, pack_t, p, kref, kc);
+}
+
+extern "C++" {
+template <typename T>
+static int unpack_t(ddx_plan* p, const void* kc, void* kref) {
+  LayoutArgs L = layout_args(p);
+  ProfScope prof(p, PC_LAYOUT, (double)p->nref * 16 + (double)p->nk * sizeof(cpx<T>));
+  hipLaunchKernelGGL((unpack_kernel<T>), dim3((p->nref + 255) / 256), dim3(256), 0, p->stream,
+                     L, (const cpx<T>*)kc, (cpx<double>*)kref);
+  LAUNCHCHK();
+  return 0;
+}
+}  // extern "C++"
 
 int ddx_unpack(ddx_plan* p, const void* kc, void* kref) {
   if (check_plan(p) || !kref || !kc) return fail(DDX_ERR_ARG, "null argument");
-  LayoutArgs L = layout_args(p);
-  hipLaunchKernelGGL((unpack_kernel<double>), dim3((p->nref + 255) / 256), dim3(256), 0, p->stream,
-                     L, (const cpx<double>*)kc, (cpx<double>*)kref);
-  LAUNCHCHK();
-  return 0;
+  return DDX_DISPATCH(p, unpack_t, p, kc, kref);
 }
 
 int ddx_fwd(ddx_plan* p, const void* x, void* kc) {
   if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
-  return fwd_impl<double>(p, (const double*)x, (cpx<double>*)kc);
+  if (p->precision == DDX_F64) return fwd_impl<double>(p, (const double*)x, (cpx<double>*)kc);
+  return fwd_impl<float>(p, (const float*)x, (cpx<float>*)kc);
 }
 
 int ddx_bwd(ddx_plan* p, const void* kc, void* x) {
   if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
-  return bwd_impl<double>(p, (const cpx<double>*)kc, (double*)x);
+  if (p->precision == DDX_F64) return bwd_impl<double>(p, (const cpx<double>*)kc, (double*)x);
+  return bwd_impl<float>(p, (const cpx<float>*)kc, (float*)x);
 }
 
 int ddx_deriv(ddx_plan* p, const void* kc, void* out, int dim) {
   if (check_plan(p) || !kc || !out) return fail(DDX_ERR_ARG, "null argument");
   if (dim < 0 || dim >= p->ndim) return fail(DDX_ERR_ARG, "bad derivative dimension");
-  hipLaunchKernelGGL((deriv_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream,
-                     kgeom(p), p->nk, dim, (const cpx<double>*)kc, (cpx<double>*)out);
+  if (p->precision == DDX_F64)
+    hipLaunchKernelGGL((deriv_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream,
+                       kgeom(p), p->nk, dim, (const cpx<double>*)kc, (cpx<double>*)out);
+  else
+    hipLaunchKernelGGL((deriv_kernel<float>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream,
+                       kgeom(p), p->nk, dim, (const cpx<float>*)kc, (cpx<float>*)out);
   LAUNCHCHK();
   return 0;
 }
 
-int ddx_project(ddx_plan* p, void* const* comps) {
-  if (check_plan(p) || !comps) return fail(DDX_ERR_ARG, "null argument");
-  ProjArgs<double> a;
+extern "C++" {
+template <typename T>
+static int project_t(ddx_plan* p, void* const* comps) {
+  ProjArgs<T> a;
   memset(&a, 0, sizeof(a));
   for (int i = 0; i < p->ndim; ++i) {
     if (!comps[i]) return fail(DDX_ERR_ARG, "null component");
-    a.c[i] = (cpx<double>*)comps[i];
+    a.c[i] = (cpx<T>*)comps[i];
   }
   a.g = kgeom(p);
   a.nk = p->nk;
   a.ndim = p->ndim;
-  hipLaunchKernelGGL((project_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
+  hipLaunchKernelGGL((project_kernel<T>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
   LAUNCHCHK();
   return 0;
+}
+}  // extern "C++"
+
+int ddx_project(ddx_plan* p, void* const* comps) {
+  if (check_plan(p) || !comps) return fail(DDX_ERR_ARG, "null argument");
+  return DDX_DISPATCH(p, project_t, p, comps);
 }
 
 int ddx_update(ddx_plan* p, int kind, const void* y, void* out, const void* F1, const void* F2,
                double coeff, int order, double dt) {
   if (check_plan(p)) return DDX_ERR_ARG;
-  const cpx<double>* yy = (const cpx<double>*)y;
-  cpx<double>* oo = (cpx<double>*)out;
-  const cpx<double>* f1 = (const cpx<double>*)F1;
-  const cpx<double>* f2 = (const cpx<double>*)F2;
-  return update_impl<double>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, order, dt);
+  if (p->precision == DDX_F64) {
+    const cpx<double>* yy = (const cpx<double>*)y;
+    cpx<double>* oo = (cpx<double>*)out;
+    const cpx<double>* f1 = (const cpx<double>*)F1;
+    const cpx<double>* f2 = (const cpx<double>*)F2;
+    return update_impl<double>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, order, dt);
+  }
+  const cpx<float>* yy = (const cpx<float>*)y;
+  cpx<float>* oo = (cpx<float>*)out;
+  const cpx<float>* f1 = (const cpx<float>*)F1;
+  const cpx<float>* f2 = (const cpx<float>*)F2;
+  return update_impl<float>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, order, dt);
 }
 
 int ddx_rhs(ddx_plan* p, const ddx_params* prm, const void* const* state, void* const* deriv,
             const void* const* forcing) {
   if (check_plan(p) || !prm || !state || !deriv) return fail(DDX_ERR_ARG, "null argument");
-  return rhs_impl<double>(p, prm, (const cpx<double>* const*)state, (cpx<double>* const*)deriv,
-                          (const cpx<double>* const*)forcing);
+  if (p->precision == DDX_F64)
+    return rhs_impl<double>(p, prm, (const cpx<double>* const*)state, (cpx<double>* const*)deriv,
+                            (const cpx<double>* const*)forcing);
+  return rhs_impl<float>(p, prm, (const cpx<float>* const*)state, (cpx<float>* const*)deriv,
+                         (const cpx<float>* const*)forcing);
 }
 
 int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
@@ -1035,58 +1088,78 @@ int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* st
   if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
   if (nsteps < 0) return fail(DDX_ERR_ARG, "negative step count");
   void* const* st = stage ? stage : state;
-  return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
-                           (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2, nsteps, dt);
+  if (p->precision == DDX_F64)
+    return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
+                             (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2, nsteps, dt);
+  return step_impl<float>(p, integrator, prm, (cpx<float>* const*)state, (cpx<float>* const*)st,
+                          (cpx<float>* const*)deriv1, (cpx<float>* const*)deriv2, nsteps, dt);
 }
+
+extern "C++" {
+template <typename T>
+static int kcombine_t(ddx_plan* p, void* out, int nterm, const void* const* src, const double* coef,
+                      const int* op, int accumulate) {
+  KCombArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  for (int t = 0; t < nterm; ++t) {
+    if (!src[t]) return fail(DDX_ERR_ARG, "null source");
+    if (op[t] < -1 || op[t] > 3 || (op[t] >= 0 && op[t] <= 2 && op[t] >= p->ndim))
+      return fail(DDX_ERR_ARG, "bad operator");
+    a.src[t] = (const cpx<T>*)src[t];
+    a.coef[t] = coef[t];
+    a.op[t] = op[t];
+  }
+  a.nterm = nterm;
+  a.accumulate = accumulate;
+  a.out = (cpx<T>*)out;
+  a.g = kgeom(p);
+  a.nk = p->nk;
+  ProfScope prof(p, PC_POINTWISE,
+                 (double)p->Ky * p->Kz * p->Kx * (nterm + 1 + accumulate) * sizeof(cpx<T>));
+  hipLaunchKernelGGL((kcombine_kernel<T>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
+  LAUNCHCHK();
+  return 0;
+}
+}  // extern "C++"
 
 int ddx_kcombine(ddx_plan* p, void* out, int nterm, const void* const* src, const double* coef,
                  const int* op, int accumulate) {
   if (check_plan(p) || !out || (nterm > 0 && (!src || !coef || !op)))
     return fail(DDX_ERR_ARG, "null argument");
   if (nterm < 0 || nterm > DDX_MAXT) return fail(DDX_ERR_ARG, "1..4 terms");
-  KCombArgs<double> a;
-  memset(&a, 0, sizeof(a));
-  for (int t = 0; t < nterm; ++t) {
-    if (!src[t]) return fail(DDX_ERR_ARG, "null source");
-    if (op[t] < -1 || op[t] > 3 || (op[t] >= 0 && op[t] <= 2 && op[t] >= p->ndim))
-      return fail(DDX_ERR_ARG, "bad operator");
-    a.src[t] = (const cpx<double>*)src[t];
-    a.coef[t] = coef[t];
-    a.op[t] = op[t];
-  }
-  a.nterm = nterm;
-  a.accumulate = accumulate;
-  a.out = (cpx<double>*)out;
-  a.g = kgeom(p);
-  a.nk = p->nk;
-  ProfScope prof(p, PC_POINTWISE, (double)p->Ky * p->Kz * p->Kx * (nterm + 1 + accumulate) * 16.0);
-  hipLaunchKernelGGL((kcombine_kernel<double>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
-  LAUNCHCHK();
-  return 0;
+  return DDX_DISPATCH(p, kcombine_t, p, out, nterm, src, coef, op, accumulate);
 }
 
-int ddx_xcombine(ddx_plan* p, void* out, int nterm, const void* const* a_, const void* const* b_,
-                 const double* coef, int accumulate) {
-  if (check_plan(p) || !out || (nterm > 0 && (!a_ || !coef))) return fail(DDX_ERR_ARG, "null argument");
-  if (nterm < 0 || nterm > DDX_MAXT) return fail(DDX_ERR_ARG, "1..4 terms");
-  XCombArgs<double> a;
+extern "C++" {
+template <typename T>
+static int xcombine_t(ddx_plan* p, void* out, int nterm, const void* const* a_, const void* const* b_,
+                      const double* coef, int accumulate) {
+  XCombArgs<T> a;
   memset(&a, 0, sizeof(a));
   int nb = 0;
   for (int t = 0; t < nterm; ++t) {
     if (!a_[t]) return fail(DDX_ERR_ARG, "null operand");
-    a.a[t] = (const double*)a_[t];
-    a.b[t] = b_ ? (const double*)b_[t] : nullptr;
+    a.a[t] = (const T*)a_[t];
+    a.b[t] = b_ ? (const T*)b_[t] : nullptr;
     nb += a.b[t] ? 1 : 0;
     a.coef[t] = coef[t];
   }
   a.nterm = nterm;
   a.accumulate = accumulate;
-  a.out = (double*)out;
+  a.out = (T*)out;
   a.n = p->npts;
-  ProfScope prof(p, PC_POINTWISE, (double)p->npts * (nterm + nb + 1 + accumulate) * 8.0);
-  hipLaunchKernelGGL((xcombine_kernel<double>), dim3((p->npts + 255) / 256), dim3(256), 0, p->stream, a);
+  ProfScope prof(p, PC_POINTWISE, (double)p->npts * (nterm + nb + 1 + accumulate) * sizeof(T));
+  hipLaunchKernelGGL((xcombine_kernel<T>), dim3((p->npts + 255) / 256), dim3(256), 0, p->stream, a);
   LAUNCHCHK();
   return 0;
+}
+}  // extern "C++"
+
+int ddx_xcombine(ddx_plan* p, void* out, int nterm, const void* const* a_, const void* const* b_,
+                 const double* coef, int accumulate) {
+  if (check_plan(p) || !out || (nterm > 0 && (!a_ || !coef))) return fail(DDX_ERR_ARG, "null argument");
+  if (nterm < 0 || nterm > DDX_MAXT) return fail(DDX_ERR_ARG, "1..4 terms");
+  return DDX_DISPATCH(p, xcombine_t, p, out, nterm, a_, b_, coef, accumulate);
 }
 
 static int finish_reduce(ddx_plan* p, int nblk, int op, double* out) {
@@ -1112,8 +1185,12 @@ int ddx_xreduce(ddx_plan* p, const void* x, int op, double* out) {
   int rc = ensure_partial(p);
   if (rc) return rc;
   const int nblk = (int)std::min<long>(1024, (p->npts + 255) / 256);
-  hipLaunchKernelGGL((xreduce_kernel<double>), dim3(nblk), dim3(256), 0, p->stream, (const double*)x,
-                     p->npts, op, p->partial);
+  if (p->precision == DDX_F64)
+    hipLaunchKernelGGL((xreduce_kernel<double>), dim3(nblk), dim3(256), 0, p->stream, (const double*)x,
+                       p->npts, op, p->partial);
+  else
+    hipLaunchKernelGGL((xreduce_kernel<float>), dim3(nblk), dim3(256), 0, p->stream, (const float*)x,
+                       p->npts, op, p->partial);
   LAUNCHCHK();
   return finish_reduce(p, nblk, op, out);
 }
@@ -1124,8 +1201,12 @@ int ddx_kreduce(ddx_plan* p, const void* k0, const void* k1, int op, double* out
   int rc = ensure_partial(p);
   if (rc) return rc;
   const int nblk = (int)std::min<long>(1024, (p->nk + 255) / 256);
-  hipLaunchKernelGGL((kreduce_kernel<double>), dim3(nblk), dim3(256), 0, p->stream,
-                     (const cpx<double>*)k0, (const cpx<double>*)k1, kgeom(p), p->nk, op, p->partial);
+  if (p->precision == DDX_F64)
+    hipLaunchKernelGGL((kreduce_kernel<double>), dim3(nblk), dim3(256), 0, p->stream,
+                       (const cpx<double>*)k0, (const cpx<double>*)k1, kgeom(p), p->nk, op, p->partial);
+  else
+    hipLaunchKernelGGL((kreduce_kernel<float>), dim3(nblk), dim3(256), 0, p->stream,
+                       (const cpx<float>*)k0, (const cpx<float>*)k1, kgeom(p), p->nk, op, p->partial);
   LAUNCHCHK();
   return finish_reduce(p, nblk, 1, out);
 }

@@ -276,9 +276,9 @@ struct EpiArgs {
   KGeom g;
   long nk;
   int physics, ndim, bdir;
-  T s;        // 1 / N
-  T galpha;   // g * alpha_t
-  T beta;
+  double npts;    // N = prod(xshape); the epilogue multiplies by 1 / N
+  double galpha;  // g * alpha_t
+  double beta;
 };
 
 template <typename T>
@@ -293,54 +293,62 @@ __device__ inline void kvec(const KGeom& g, long idx, int& col, T& kx, T& ky, T&
 }
 
 template <typename T>
+__device__ inline cpx<double> c64(cpx<T> v) { return mk(double(v.x), double(v.y)); }
+template <typename T>
+__device__ inline cpx<T> cT(cpx<double> v) { return mk(T(v.x), T(v.y)); }
+
+// k-space epilogue of the RHS, computed in FP64 whatever the storage type
+// (projection and curl in double precision).
+template <typename T>
 __global__ void epilogue_kernel(const EpiArgs<T> a) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.nk) return;
   int col;
-  T kx, ky, kz;
-  kvec<T>(a.g, idx, col, kx, ky, kz);
+  double kx, ky, kz;
+  kvec<double>(a.g, idx, col, kx, ky, kz);
   const int nd = a.ndim;
   const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
   if (col >= a.g.Kx) {
     for (int v = 0; v < nvar; ++v) a.deriv[v][idx] = mk(T(0), T(0));
     return;
   }
-  const T kk[3] = {kx, ky, kz};
-  cpx<T> F[3];
+  const double s = 1.0 / a.npts;
+  const double kk[3] = {kx, ky, kz};
+  cpx<double> F[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    F[i] = mk(T(0), T(0));
+    F[i] = mk(0.0, 0.0);
     if (i < nd) {
-      F[i] = scale(a.U[i][idx], a.s);
-      if (a.forcing[i]) F[i] = F[i] + a.forcing[i][idx];
-      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(a.state[nd][idx], a.galpha);
+      F[i] = scale(c64(a.U[i][idx]), s);
+      if (a.forcing[i]) F[i] = F[i] + c64(a.forcing[i][idx]);
+      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(c64(a.state[nd][idx]), a.galpha);
     }
   }
   // projection F <- F - k (k.F) / |k|^2, |k|^2 = 0 replaced by 1
-  T k2 = kx * kx + ky * ky + kz * kz;
-  if (k2 == T(0)) k2 = T(1);
-  cpx<T> kf = mk(T(0), T(0));
+  double k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 == 0.0) k2 = 1.0;
+  cpx<double> kf = mk(0.0, 0.0);
 #pragma unroll
   for (int i = 0; i < 3; ++i) kf = kf + scale(F[i], kk[i]);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-    if (i < nd) a.deriv[i][idx] = F[i] - scale(kf, kk[i] / k2);
+    if (i < nd) a.deriv[i][idx] = cT<T>(F[i] - scale(kf, kk[i] / k2));
   if (a.physics == PH_BOUSS) {
-    cpx<T> ft = scale(a.U[nd][idx], a.s) - scale(a.state[a.bdir][idx], a.beta);
-    if (a.forcing[nd]) ft = ft + a.forcing[nd][idx];
-    a.deriv[nd][idx] = ft;
+    cpx<double> ft = scale(c64(a.U[nd][idx]), s) - scale(c64(a.state[a.bdir][idx]), a.beta);
+    if (a.forcing[nd]) ft = ft + c64(a.forcing[nd][idx]);
+    a.deriv[nd][idx] = cT<T>(ft);
   } else if (a.physics == PH_MHD) {
     if (nd == 3) {
-      const cpx<T> e0 = scale(a.U[3][idx], a.s), e1 = scale(a.U[4][idx], a.s),
-                   e2 = scale(a.U[5][idx], a.s);
+      const cpx<double> e0 = scale(c64(a.U[3][idx]), s), e1 = scale(c64(a.U[4][idx]), s),
+                        e2 = scale(c64(a.U[5][idx]), s);
       // i k x E
-      a.deriv[3][idx] = mul_is(e2, ky) - mul_is(e1, kz);
-      a.deriv[4][idx] = mul_is(e0, kz) - mul_is(e2, kx);
-      a.deriv[5][idx] = mul_is(e1, kx) - mul_is(e0, ky);
+      a.deriv[3][idx] = cT<T>(mul_is(e2, ky) - mul_is(e1, kz));
+      a.deriv[4][idx] = cT<T>(mul_is(e0, kz) - mul_is(e2, kx));
+      a.deriv[5][idx] = cT<T>(mul_is(e1, kx) - mul_is(e0, ky));
     } else {
-      const cpx<T> e = scale(a.U[2][idx], a.s);
-      a.deriv[2][idx] = mul_is(e, ky);
-      a.deriv[3][idx] = mk(T(0), T(0)) - mul_is(e, kx);
+      const cpx<double> e = scale(c64(a.U[2][idx]), s);
+      a.deriv[2][idx] = cT<T>(mul_is(e, ky));
+      a.deriv[3][idx] = cT<T>(mk(0.0, 0.0) - mul_is(e, kx));
     }
   }
 }
@@ -386,9 +394,9 @@ __global__ void update_kernel(const UpdArgs<T> a) {
     return;
   }
   const double dt = a.dt;
-  const cpx<T> y = a.y[comp][idx];
-  const cpx<T> f1 = a.F1[comp][idx];
-  const cpx<T> f2 = (a.kind >= UPD_ETD2RK1) ? a.F2[comp][idx] : mk(T(0), T(0));
+  const cpx<double> y = c64(a.y[comp][idx]);
+  const cpx<double> f1 = c64(a.F1[comp][idx]);
+  const cpx<double> f2 = (a.kind >= UPD_ETD2RK1) ? c64(a.F2[comp][idx]) : mk(0.0, 0.0);
   double Z = 0.0;
   const double c = a.coeff[comp];
   if (c != 0.0 && a.kind != UPD_EULER) {
@@ -399,11 +407,11 @@ __global__ void update_kernel(const UpdArgs<T> a) {
     else kp = pow(k2, (double)a.order);
     Z = (-(c * kp)) * dt;
   }
-  cpx<T> res;
+  cpx<double> res;
   if (Z == 0.0) {
-    if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, T(dt / 2.0));
-    else if (a.kind == UPD_ETD2RK2) res = y + scale(f2, T(dt));
-    else res = y + scale(f1, T(dt));
+    if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, dt / 2.0);
+    else if (a.kind == UPD_ETD2RK2) res = y + scale(f2, dt);
+    else res = y + scale(f1, dt);
   } else {
     double p0, p1, p2;
     if (Z < 0.5 && Z > -0.5) {
@@ -415,11 +423,11 @@ __global__ void update_kernel(const UpdArgs<T> a) {
       p1 = (p0 - 1.0) / Z;
       p2 = (p1 - 1.0) / Z;
     }
-    if (a.kind == UPD_ETD1) res = scale(y, T(p0)) + scale(f1, T(p1 * dt));
-    else if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, T(p2 * dt));
-    else res = scale(y, T(p0)) + scale(f2 - f1, T(2.0 * p2 * dt)) + scale(f1, T(p1 * dt));
+    if (a.kind == UPD_ETD1) res = scale(y, p0) + scale(f1, p1 * dt);
+    else if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, p2 * dt);
+    else res = scale(y, p0) + scale(f2 - f1, 2.0 * p2 * dt) + scale(f1, p1 * dt);
   }
-  a.out[comp][idx] = res;
+  a.out[comp][idx] = cT<T>(res);
 }
 
 // layout conversion: kref (reference transposed half spectrum) <-> compact
@@ -439,7 +447,7 @@ __device__ inline long ref_index(const LayoutArgs& L, int m, int fy, int fz) {
 }
 
 template <typename T>
-__global__ void pack_kernel(const LayoutArgs L, const cpx<T>* ref, cpx<T>* kc) {
+__global__ void pack_kernel(const LayoutArgs L, const cpx<double>* ref, cpx<T>* kc) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= L.nk) return;
   const int col = (int)(idx % L.Kxp);
@@ -449,22 +457,26 @@ __global__ void pack_kernel(const LayoutArgs L, const cpx<T>* ref, cpx<T>* kc) {
   if (col >= L.Kx) { kc[idx] = mk(T(0), T(0)); return; }
   const int fy = slab <= L.cy ? slab : slab + (L.ny - L.Ky);
   const int fz = row <= L.cz ? row : row + (L.nz - L.Kz);
-  kc[idx] = ref[ref_index(L, col, fy, fz)];
+  const cpx<double> v = ref[ref_index(L, col, fy, fz)];
+  kc[idx] = mk(T(v.x), T(v.y));
 }
 
 template <typename T>
-__global__ void unpack_kernel(const LayoutArgs L, const cpx<T>* kc, cpx<T>* ref) {
+__global__ void unpack_kernel(const LayoutArgs L, const cpx<T>* kc, cpx<double>* ref) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= L.nref) return;
   const int hx = L.nx / 2 + 1;
   int m, fy, fz;
   if (L.ndim == 2) { m = (int)(idx / L.ny); fy = (int)(idx % L.ny); fz = 0; }
   else { m = (int)(idx % hx); long t = idx / hx; fz = (int)(t % L.nz); fy = (int)(t / L.nz); }
-  cpx<T> v = mk(T(0), T(0));
+  cpx<double> v = mk(0.0, 0.0);
   int row = -1, slab = -1;
   if (fy <= L.cy) slab = fy; else if (fy >= L.ny - L.cy) slab = fy - (L.ny - L.Ky);
   if (fz <= L.cz) row = fz; else if (fz >= L.nz - L.cz) row = fz - (L.nz - L.Kz);
-  if (m < L.Kx && slab >= 0 && row >= 0) v = kc[((long)slab * L.Kz + row) * L.Kxp + m];
+  if (m < L.Kx && slab >= 0 && row >= 0) {
+    const cpx<T> u = kc[((long)slab * L.Kz + row) * L.Kxp + m];
+    v = mk(double(u.x), double(u.y));
+  }
   ref[idx] = v;
 }
 
@@ -496,12 +508,12 @@ __global__ void project_kernel(const ProjArgs<T> a) {
   T kx, ky, kz;
   kvec<T>(a.g, idx, col, kx, ky, kz);
   if (col >= a.g.Kx) return;
-  const T kk[3] = {kx, ky, kz};
-  T k2 = kx * kx + ky * ky + kz * kz;
-  if (k2 == T(0)) k2 = T(1);
-  cpx<T> kf = mk(T(0), T(0));
-  for (int i = 0; i < a.ndim; ++i) kf = kf + scale(a.c[i][idx], kk[i]);
-  for (int i = 0; i < a.ndim; ++i) a.c[i][idx] = a.c[i][idx] - scale(kf, kk[i] / k2);
+  const double kk[3] = {double(kx), double(ky), double(kz)};
+  double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+  if (k2 == 0.0) k2 = 1.0;
+  cpx<double> kf = mk(0.0, 0.0);
+  for (int i = 0; i < a.ndim; ++i) kf = kf + scale(c64(a.c[i][idx]), kk[i]);
+  for (int i = 0; i < a.ndim; ++i) a.c[i][idx] = cT<T>(c64(a.c[i][idx]) - scale(kf, kk[i] / k2));
 }
 
 // out (+)= sum_t coef_t * op_t(k) * src_t on compact k arrays.
@@ -630,6 +642,13 @@ __global__ void kreduce_kernel(const cpx<T>* s0, const cpx<T>* s1, KGeom g, long
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// real array conversion (x-space staging between the FP64 host boundary and FP32 plans)
+template <typename TI, typename TO>
+__global__ void convert_kernel(const TI* in, TO* out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = TO(in[i]);
 }
 
 __global__ void twiddle_kernel(cpx<double>* tw, int n) {

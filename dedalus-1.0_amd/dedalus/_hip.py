@@ -276,15 +276,22 @@ class Plan:
         return out
 
     def upload_x(self, xd, x):
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        """Real x-space array (host) -> device, in the plan's precision."""
+        x = np.ascontiguousarray(x, dtype=np.float64 if self.precision == F64 else np.float32)
         if x.shape != self.shape:
             raise ValueError("x array shape %s, expected %s" % (x.shape, self.shape))
         self.h2d(xd, x)
 
     def download_x(self, xd, out=None):
+        """Device x-space array -> float64 host array."""
         if out is None:
             out = np.empty(self.shape, dtype=np.float64)
-        self.d2h(out, xd)
+        if self.precision == F64:
+            self.d2h(out, xd)
+        else:
+            tmp = np.empty(self.shape, dtype=np.float32)
+            self.d2h(tmp, xd)
+            out[...] = tmp
         return out
 
     # --- numerics ---------------------------------------------------------

@@ -35,5 +35,6 @@ decfg.set('analysis', 'powerspectrum_dpi', '100')
 
 decfg.add_section('hip')
 decfg.set('hip', 'device', '0')
+decfg.set('hip', 'precision', 'float64')   # float32: FP32 transforms and products
 
 decfg.read([os.path.expanduser('~/.dedalus/config'), 'dedalus.cfg'])

@@ -110,8 +110,12 @@ class FourierRepresentation(Representation):
                             'xspace': self.global_shape['xspace'].copy()}
         self.offset = {'xspace': 0, 'kspace': 0}
         _hip.init(decfg.getint('hip', 'device'))
+        prec = decfg.get('hip', 'precision')
+        if prec not in ('float64', 'float32'):
+            raise ValueError("[hip] precision must be float64 or float32")
         self._plan = _hip.get_plan(tuple(int(n) for n in self.global_shape['xspace']),
-                                   tuple(float(x) for x in self.length))
+                                   tuple(float(x) for x in self.length),
+                                   _hip.F64 if prec == 'float64' else _hip.F32)
         self._kc = self._plan.alloc_k()
         self._xd = None
         self._dd = None
