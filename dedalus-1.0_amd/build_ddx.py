@@ -24,8 +24,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["ddx.hip"]
-HEADERS = ["ddx_kernels.h", "fft_core.h"]
+SOURCES = ["ddx.hip", "ddx_closure.hip"]
+HEADERS = ["ddx_kernels.h", "ddx_fast.h", "fft_core.h"]
 
 
 def build(verbose=False):

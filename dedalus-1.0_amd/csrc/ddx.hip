@@ -117,6 +117,10 @@ struct ddx_plan {
   double* partial = nullptr;  // reduction partials (device)
   std::vector<double> hpartial;
   bool generic_only = false;  // DDX_FORCE_GENERIC=1: size-generic kernels everywhere
+  // one RK step captured as a HIP graph and replayed (DDX_NO_GRAPH=1 disables)
+  bool use_graph = true;
+  hipGraphExec_t step_exec = nullptr;
+  std::vector<unsigned char> step_key;
   bool prof = false;
   struct Rec { int cat; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
@@ -685,7 +689,8 @@ void comp_coeffs(const ddx_params* prm, int nd, int nstate, double* coeff) {
 
 template <typename T>
 int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const* y,
-              cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2, int nsteps, double dt) {
+              cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2,
+              const cpx<T>* const* forcing, int nsteps, double dt) {
   const int nd = p->ndim;
   const int ns = nstate_of(prm, nd);
   if (ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
@@ -698,22 +703,74 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
   double coeff[DDX_MAXF];
   comp_coeffs(prm, nd, ns, coeff);
   const int vo = prm->viscosity_order;
-  int rc;
-  for (int s = 0; s < nsteps; ++s) {
+  auto one_step = [&]() -> int {
+    int rc;
     if (integrator == DDX_RK2MID) {
       // time_step.py:285-305
-      if ((rc = rhs_impl<T>(p, prm, y, d1, nullptr))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, stage, d1, nullptr, coeff, vo, dt / 2.0))) return rc;
-      if ((rc = rhs_impl<T>(p, prm, stage, d2, nullptr))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, stage, d2, forcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK2, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     } else {
       // time_step.py:369-389
-      if ((rc = rhs_impl<T>(p, prm, y, d1, nullptr))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, y, d1, nullptr, coeff, vo, dt))) return rc;
-      if ((rc = rhs_impl<T>(p, prm, y, d2, nullptr))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d2, forcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK1, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     }
+    return 0;
+  };
+  int rc;
+  if (!p->use_graph || p->prof || nsteps < 1) {
+    for (int s = 0; s < nsteps; ++s)
+      if ((rc = one_step())) return rc;
+    return 0;
   }
+  // Capture one step as a graph keyed by everything it depends on; replay it.
+  PhysLayout L;
+  phys_layout(prm->physics, nd, &L);
+  if ((rc = ensure_ws(p, std::max(L.nq, L.np)))) return rc;
+  std::vector<unsigned char> key;
+  auto add = [&key](const void* q, size_t n) {
+    const unsigned char* c = (const unsigned char*)q;
+    key.insert(key.end(), c, c + n);
+  };
+  add(&integrator, sizeof(integrator));
+  add(prm, sizeof(*prm));
+  add(&dt, sizeof(dt));
+  add(y, sizeof(void*) * ns);
+  add(stage, sizeof(void*) * ns);
+  add(d1, sizeof(void*) * ns);
+  add(d2, sizeof(void*) * ns);
+  for (int i = 0; i < ns; ++i) {
+    const void* f = forcing ? (const void*)forcing[i] : nullptr;
+    add(&f, sizeof(f));
+  }
+  add(&p->wsA, sizeof(void*));
+  add(&p->stream, sizeof(void*));
+  if (!p->step_exec || key != p->step_key) {
+    if (p->step_exec) {
+      HIPCHK(hipGraphExecDestroy(p->step_exec));
+      p->step_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    const int crc = one_step();
+    const hipError_t ec = hipStreamEndCapture(p->stream, &g);
+    if (crc) {
+      if (g) (void)hipGraphDestroy(g);
+      return crc;
+    }
+    if (ec != hipSuccess) return fail(DDX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&p->step_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      p->step_exec = nullptr;
+      return fail(DDX_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    p->step_key = key;
+  }
+  for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(p->step_exec, p->stream));
   return 0;
 }
 
@@ -740,6 +797,11 @@ int check_plan(const ddx_plan* p) {
 }
 
 }  // namespace
+
+namespace ddx {
+// error reporting shared with the closure kernels' entry points
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace ddx
 
 // ===========================================================================
 // C ABI
@@ -779,6 +841,8 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   {
     const char* env = getenv("DDX_FORCE_GENERIC");
     p->generic_only = env && env[0] == '1';
+    const char* ng = getenv("DDX_NO_GRAPH");
+    p->use_graph = !(ng && ng[0] == '1');
   }
   p->ndim = ndim;
   p->precision = precision;
@@ -870,6 +934,7 @@ int ddx_plan_destroy(ddx_plan* p) {
   if (p->wsA) (void)hipFree(p->wsA);
   if (p->wsB) (void)hipFree(p->wsB);
   if (p->partial) (void)hipFree(p->partial);
+  if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
   for (auto& r : p->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (hipEvent_t e : p->evfree) (void)hipEventDestroy(e);
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
@@ -896,6 +961,17 @@ int ddx_plan_ktable(const ddx_plan* p, int axis, double* out, int64_t n) {
   if (!a || !out) return fail(DDX_ERR_ARG, "bad axis");
   if (n != a->K) return fail(DDX_ERR_ARG, "ktable length mismatch");
   for (int j = 0; j < a->K; ++j) out[j] = a->kfull[(axis == 0 || j <= a->c) ? j : j + a->n - a->K];
+  return 0;
+}
+
+int ddx_plan_set_graph(ddx_plan* p, int on) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  if (p->step_exec) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipGraphExecDestroy(p->step_exec));
+    p->step_exec = nullptr;
+  }
+  p->use_graph = on != 0;
   return 0;
 }
 
@@ -1083,16 +1159,19 @@ int ddx_rhs(ddx_plan* p, const ddx_params* prm, const void* const* state, void* 
 }
 
 int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
-             void* const* stage, void* const* deriv1, void* const* deriv2, int nsteps, double dt) {
+             void* const* stage, void* const* deriv1, void* const* deriv2,
+             const void* const* forcing, int nsteps, double dt) {
   if (check_plan(p) || !prm || !state || !deriv1 || !deriv2) return fail(DDX_ERR_ARG, "null argument");
   if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
   if (nsteps < 0) return fail(DDX_ERR_ARG, "negative step count");
   void* const* st = stage ? stage : state;
   if (p->precision == DDX_F64)
     return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
-                             (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2, nsteps, dt);
+                             (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2,
+                             (const cpx<double>* const*)forcing, nsteps, dt);
   return step_impl<float>(p, integrator, prm, (cpx<float>* const*)state, (cpx<float>* const*)st,
-                          (cpx<float>* const*)deriv1, (cpx<float>* const*)deriv2, nsteps, dt);
+                          (cpx<float>* const*)deriv1, (cpx<float>* const*)deriv2,
+                          (const cpx<float>* const*)forcing, nsteps, dt);
 }
 
 extern "C++" {

@@ -1,0 +1,473 @@
+// Learned sub-grid closure kernels (gfx950): BF16 MFMA 3x3 periodic
+// convolutions with fused epilogues, and the data movement around them.
+//
+// Layouts
+//   activations  [npix][Cp]        bf16 bit patterns, npix = B*H*W (b, y, x
+//                                  order), channels padded to multiples of 16
+//   weights      [Co][9][Ci]       bf16, tap t = 3*(dy+1) + (dx+1)
+//   pre-activations / gradients    fp32, same pixel-major layout
+// The domain is periodic in y and x, so taps wrap around.
+//
+// v_mfma_f32_16x16x16_bf16 per wavefront (D = A B, 16 x 16 x 16):
+//   A (rows m, cols k): lane l holds A[l % 16][4*(l/16) + i], i < 4
+//   B (rows k, cols n): lane l holds B[4*(l/16) + i][l % 16]
+//   D:                  lane l holds D[4*(l/16) + i][l % 16]
+// ddx_mfma_probe runs one tile with exactly these placements so the layout
+// can be checked against a host product.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/ddx.h"
+#include "../../include/ddx_closure.h"
+
+namespace ddx {
+int set_error(int code, const std::string& msg);  // defined in ddx.hip
+}
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define CL_LAUNCHCHK(name)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                      \
+    if (e_ != hipSuccess)                                                                   \
+      return ddx::set_error(DDX_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// fp32 -> bf16, round to nearest even; NaN stays a (quiet) NaN
+__device__ inline uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU with the normal CDF Phi(z) = erfc(-z / sqrt 2) / 2 (erfc keeps the
+// negative tail free of cancellation): GELU(z) = z Phi(z), GELU'(z) = Phi(z) + z phi(z)
+__device__ inline float gelu(float z) { return 0.5f * z * erfcf(-z * 0.70710678118654752f); }
+__device__ inline float gelu_grad(float z) {
+  return 0.5f * erfcf(-z * 0.70710678118654752f) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__device__ inline bf16x4 load4(const uint16_t* p) {
+  // four consecutive channels; channel offsets are multiples of 4, so 8-byte aligned
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xffffu);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffffu);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+__device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+struct ConvArgs {
+  const uint16_t* x;  // [npix][Ci]
+  const uint16_t* w;  // [Co][9][Ci]
+  const float* bias;  // [Co] or null
+  const float* aux;   // [npix][Co] pre-activation for DDX_CONV_GELU_GRAD
+  float* out_f32;     // [npix][Co] or null
+  uint16_t* out_bf16; // [npix][Co] or null
+  int B, H, W, Ci, Co, mode;
+};
+
+// One wavefront: 16 consecutive pixels x all Co = 16*NCO output channels.
+// Four wavefronts per block; no block-level synchronisation.
+template <int NCO>
+__global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs a) {
+  const int lane = threadIdx.x & 63;
+  const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long npix = (long)a.B * a.H * a.W;
+  if (tile * 16 >= npix) return;
+  const long p0 = tile * 16;
+  const int m = lane & 15, kq = lane >> 4;
+  const long pm = p0 + m;
+  const long hw = (long)a.H * a.W;
+  const int b = (int)(pm / hw);
+  const int rem = (int)(pm - (long)b * hw);
+  const int py = rem / a.W, px = rem - py * a.W;
+  f32x4 acc[NCO];
+#pragma unroll
+  for (int j = 0; j < NCO; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 9; ++t) {
+    const int yy = wrapi(py + t / 3 - 1, a.H), xx = wrapi(px + t % 3 - 1, a.W);
+    const uint16_t* xrow = a.x + (((long)b * a.H + yy) * a.W + xx) * a.Ci + 4 * kq;
+    const uint16_t* wrow = a.w + ((long)m * 9 + t) * a.Ci + 4 * kq;
+    const long wstride = 16L * 9 * a.Ci;  // next 16 output channels
+    for (int kc = 0; kc < a.Ci; kc += 16) {
+      const bf16x4 av = load4(xrow + kc);
+#pragma unroll
+      for (int j = 0; j < NCO; ++j) {
+        const bf16x4 bv = load4(wrow + j * wstride + kc);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NCO; ++j) {
+    const int c = j * 16 + m;
+    const float bv = a.bias ? a.bias[c] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long o = (p0 + 4 * kq + i) * a.Co + c;
+      float v = acc[j][i];
+      float act;
+      if (a.mode == DDX_CONV_GELU_GRAD) {
+        v *= gelu_grad(a.aux[o]);
+        act = v;
+      } else {
+        v += bv;
+        act = a.mode == DDX_CONV_GELU ? gelu(v) : v;
+      }
+      if (a.out_f32) a.out_f32[o] = v;
+      if (a.out_bf16) a.out_bf16[o] = f2bf(act);
+    }
+  }
+}
+
+// Weight-gradient partials over one chunk of pixels, all nine taps per wavefront:
+// partial[chunk][co][t][ci] = sum_{p in chunk} dz[p][co] x[shift_t(p)][ci].
+// grid.x = chunks, grid.y = (Co/16) * (Ci/16); one wavefront per block.
+__global__ void __launch_bounds__(64) conv3x3_wgrad_kernel(const uint16_t* __restrict__ x,
+                                                            const uint16_t* __restrict__ dz,
+                                                            float* __restrict__ partial, int B,
+                                                            int H, int W, int Ci, int Co,
+                                                            int pch) {
+  const int lane = threadIdx.x;
+  const int m = lane & 15, kq = lane >> 4;
+  const int nct = Ci / 16;
+  const int co0 = (blockIdx.y / nct) * 16, ci0 = (blockIdx.y % nct) * 16;
+  const long hw = (long)H * W;
+  const long npix = (long)B * hw;
+  const long pstart = (long)blockIdx.x * pch;
+  const long pend = pstart + pch < npix ? pstart + pch : npix;
+  f32x4 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long pk = pstart; pk < pend; pk += 16) {
+    bf16x4 av;
+    int pb[4], py[4], px[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long p = pk + 4 * kq + i;
+      av[i] = (short)dz[p * Co + co0 + m];  // A[m = co][k = p]
+      pb[i] = (int)(p / hw);
+      const int rem = (int)(p - (long)pb[i] * hw);
+      py[i] = rem / W;
+      px[i] = rem - py[i] * W;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      bf16x4 bv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int yy = wrapi(py[i] + t / 3 - 1, H), xx = wrapi(px[i] + t % 3 - 1, W);
+        bv[i] = (short)x[(((long)pb[i] * H + yy) * W + xx) * Ci + ci0 + m];  // B[k = p][n = ci]
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int co = co0 + 4 * kq + i, ci = ci0 + m;
+      partial[(((long)blockIdx.x * Co + co) * 9 + t) * Ci + ci] = acc[t][i];
+    }
+}
+
+// out[j] = sum over chunks of partial[c][j], in a fixed order (double accumulation)
+__global__ void sum_chunks_kernel(const float* __restrict__ partial, float* __restrict__ out,
+                                  int nchunk, long n) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += partial[(long)c * n + j];
+  out[j] = (float)s;
+}
+
+// Column sums of a [npix][C] fp32 array (bias gradient); one block per column
+__global__ void __launch_bounds__(256) colsum_kernel(const float* __restrict__ a,
+                                                     float* __restrict__ out, long npix, int C) {
+  __shared__ double sh[256];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (long p = threadIdx.x; p < npix; p += 256) s += a[p * C + c];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = (float)sh[0];
+}
+
+// fp32 weights in (co, ci, ky, kx) order -> padded bf16 forward and transposed
+// (input-gradient) weights: wf[co][t][ci] = w[co][ci][t], wt[ci][t][co] = w[co][ci][8 - t]
+__global__ void wprep_kernel(const float* __restrict__ w, int Co, int Ci, int Cop, int Cip,
+                             uint16_t* __restrict__ wf, uint16_t* __restrict__ wt) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)Cop * 9 * Cip;
+  if (j >= n) return;
+  const int ci = (int)(j % Cip);
+  const int t = (int)((j / Cip) % 9);
+  const int co = (int)(j / (9L * Cip));
+  const float v = (co < Co && ci < Ci) ? w[((long)co * Ci + ci) * 9 + t] : 0.f;
+  const uint16_t h = f2bf(v);
+  if (wf) wf[j] = h;
+  if (wt) wt[((long)ci * 9 + (8 - t)) * Cop + co] = h;
+}
+
+// padded weight gradient [Cop][9][Cip] -> (co, ci, ky, kx) order
+__global__ void wunpack_kernel(const float* __restrict__ dwp, int Co, int Ci, int Cop, int Cip,
+                               float* __restrict__ dw) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)Co * Ci * 9;
+  if (j >= n) return;
+  const int t = (int)(j % 9);
+  const int ci = (int)((j / 9) % Ci);
+  const int co = (int)(j / (9L * Ci));
+  dw[j] = dwp[((long)co * 9 + t) * Cip + ci];
+}
+
+// Gather channels: dst_bf16[p][c] = scale_c * src_c[p * pstride] (0 for c >= C)
+struct PackArgs {
+  const void* src[DDX_CONV_MAXCH];
+  float scale[DDX_CONV_MAXCH];
+  long pstride;
+  long npix;
+  int C, Cp, f64;
+  uint16_t* dst_bf16;
+  float* dst_f32;
+};
+
+__global__ void pack_kernel(PackArgs a) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.npix * a.Cp) return;
+  const long p = j / a.Cp;
+  const int c = (int)(j - p * a.Cp);
+  float v = 0.f;
+  if (c < a.C) {
+    const long o = p * a.pstride;
+    v = a.f64 ? (float)((const double*)a.src[c])[o] : ((const float*)a.src[c])[o];
+    v *= a.scale[c];
+  }
+  if (a.dst_bf16) a.dst_bf16[j] = f2bf(v);
+  if (a.dst_f32) a.dst_f32[j] = v;
+}
+
+// Scatter channels: dst_c[p * pstride] = scale_c * src[p][c] (c < C), fp32 or fp64
+struct UnpackArgs {
+  void* dst[DDX_CONV_MAXCH];
+  float scale[DDX_CONV_MAXCH];
+  long pstride;
+  long npix;
+  int C, Cp, f64;
+  const float* src;
+};
+
+__global__ void unpack_kernel(UnpackArgs a) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.npix * a.C) return;
+  const long p = j / a.C;
+  const int c = (int)(j - p * a.C);
+  const float v = a.src[p * a.Cp + c] * a.scale[c];
+  const long o = p * a.pstride;
+  if (a.f64)
+    ((double*)a.dst[c])[o] = (double)v;
+  else
+    ((float*)a.dst[c])[o] = v;
+}
+
+// Box filter: dst[Y][X] = mean of the f x f block of src starting at (f Y, f X)
+template <typename T>
+__global__ void box_filter_kernel(const T* __restrict__ src, T* __restrict__ dst, int ny, int nx,
+                                  int f) {
+  const int mx = nx / f, my = ny / f;
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= (long)mx * my) return;
+  const int Y = (int)(j / mx), X = (int)(j - (long)Y * mx);
+  double s = 0.0;
+  for (int a = 0; a < f; ++a) {
+    const T* row = src + ((long)(Y * f + a)) * nx + (long)X * f;
+    for (int b = 0; b < f; ++b) s += (double)row[b];
+  }
+  dst[j] = (T)(s / ((double)f * f));
+}
+
+// Layout probe: D = A B for one 16 x 16 x 16 tile; A, B bf16 row-major, D fp32 row-major.
+__global__ void __launch_bounds__(64) mfma_probe_kernel(const uint16_t* A, const uint16_t* Bm, float* D) {
+  const int lane = threadIdx.x;
+  const int m = lane & 15, kq = lane >> 4;
+  bf16x4 a, b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    a[i] = (short)A[m * 16 + 4 * kq + i];
+    b[i] = (short)Bm[(4 * kq + i) * 16 + m];
+  }
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) D[(4 * kq + i) * 16 + m] = acc[i];
+}
+
+int check_conv(int B, int H, int W, int Ci, int Co) {
+  if (B < 1 || H < 1 || W < 1) return ddx::set_error(DDX_ERR_ARG, "empty convolution");
+  if (Ci < 16 || Co < 16 || Ci % 16 || Co % 16)
+    return ddx::set_error(DDX_ERR_ARG, "padded channel counts must be positive multiples of 16");
+  if (Co > 64) return ddx::set_error(DDX_ERR_UNSUPPORTED, "at most 64 output channels");
+  if (((long)B * H * W) % 16) return ddx::set_error(DDX_ERR_ARG, "B*H*W must be a multiple of 16");
+  if ((long)B * H * W * (Ci > Co ? Ci : Co) >= (1L << 40)) return ddx::set_error(DDX_ERR_ARG, "problem too large");
+  return 0;
+}
+
+inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+extern "C" {
+
+int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x, const void* w,
+                const float* bias, const float* aux, int mode, float* out_f32, void* out_bf16) {
+  int rc = check_conv(B, H, W, Ci, Co);
+  if (rc) return rc;
+  if (!x || !w || (!out_f32 && !out_bf16)) return ddx::set_error(DDX_ERR_ARG, "null argument");
+  if (mode != DDX_CONV_LINEAR && mode != DDX_CONV_GELU && mode != DDX_CONV_GELU_GRAD)
+    return ddx::set_error(DDX_ERR_ARG, "unknown convolution mode");
+  if (mode == DDX_CONV_GELU_GRAD && !aux)
+    return ddx::set_error(DDX_ERR_ARG, "DDX_CONV_GELU_GRAD needs the pre-activation");
+  ConvArgs a{(const uint16_t*)x, (const uint16_t*)w, bias, aux, out_f32, (uint16_t*)out_bf16,
+             B, H, W, Ci, Co, mode};
+  const long tiles = (long)B * H * W / 16;
+  const dim3 grid(nblk(tiles, 4));
+  hipStream_t s = (hipStream_t)stream;
+  switch (Co / 16) {
+    case 1: hipLaunchKernelGGL(conv3x3_kernel<1>, grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(conv3x3_kernel<2>, grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(conv3x3_kernel<3>, grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL(conv3x3_kernel<4>, grid, dim3(256), 0, s, a); break;
+  }
+  CL_LAUNCHCHK("conv3x3");
+  return 0;
+}
+
+int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const void* x,
+                      const void* dz, float* partial, int nchunk, float* dw) {
+  int rc = check_conv(B, H, W, Ci, Co);
+  if (rc) return rc;
+  if (!x || !dz || !partial || !dw || nchunk < 1) return ddx::set_error(DDX_ERR_ARG, "null argument");
+  const long npix = (long)B * H * W;
+  long pch = (npix + nchunk - 1) / nchunk;
+  pch = (pch + 15) / 16 * 16;
+  const int nch = (int)((npix + pch - 1) / pch);  // <= nchunk
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(nch, (Co / 16) * (Ci / 16)), dim3(64), 0, s,
+                     (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W, Ci, Co, (int)pch);
+  CL_LAUNCHCHK("conv3x3_wgrad");
+  const long n = (long)Co * 9 * Ci;
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, partial, dw, nch, n);
+  CL_LAUNCHCHK("conv3x3_wgrad sum");
+  return 0;
+}
+
+int ddx_conv_colsum(void* stream, const float* a, int64_t npix, int C, float* out) {
+  if (!a || !out || npix < 1 || C < 1) return ddx::set_error(DDX_ERR_ARG, "bad column-sum arguments");
+  hipLaunchKernelGGL(colsum_kernel, dim3(C), dim3(256), 0, (hipStream_t)stream, a, out, (long)npix, C);
+  CL_LAUNCHCHK("conv_colsum");
+  return 0;
+}
+
+int ddx_conv_wprep(void* stream, const float* w, int Co, int Ci, int Cop, int Cip, void* wf, void* wt) {
+  if (!w || (!wf && !wt) || Co < 1 || Ci < 1 || Cop < Co || Cip < Ci)
+    return ddx::set_error(DDX_ERR_ARG, "bad weight-prep arguments");
+  const long n = (long)Cop * 9 * Cip;
+  hipLaunchKernelGGL(wprep_kernel, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, w, Co, Ci,
+                     Cop, Cip, (uint16_t*)wf, (uint16_t*)wt);
+  CL_LAUNCHCHK("conv_wprep");
+  return 0;
+}
+
+int ddx_conv_wunpack(void* stream, const float* dwp, int Co, int Ci, int Cop, int Cip, float* dw) {
+  if (!dwp || !dw || Co < 1 || Ci < 1 || Cop < Co || Cip < Ci)
+    return ddx::set_error(DDX_ERR_ARG, "bad weight-unpack arguments");
+  const long n = (long)Co * Ci * 9;
+  hipLaunchKernelGGL(wunpack_kernel, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, dwp, Co,
+                     Ci, Cop, Cip, dw);
+  CL_LAUNCHCHK("conv_wunpack");
+  return 0;
+}
+
+int ddx_conv_pack(void* stream, int64_t npix, int C, int Cp, const void* const* src, int64_t pstride,
+                  int src_f64, const float* scale, void* dst_bf16, float* dst_f32) {
+  if (!src || npix < 1 || C < 1 || C > DDX_CONV_MAXCH || Cp < C || pstride < 1 || (!dst_bf16 && !dst_f32))
+    return ddx::set_error(DDX_ERR_ARG, "bad pack arguments");
+  PackArgs a{};
+  for (int c = 0; c < C; ++c) {
+    if (!src[c]) return ddx::set_error(DDX_ERR_ARG, "null channel source");
+    a.src[c] = src[c];
+    a.scale[c] = scale ? scale[c] : 1.0f;
+  }
+  a.pstride = pstride;
+  a.npix = npix;
+  a.C = C;
+  a.Cp = Cp;
+  a.f64 = src_f64 ? 1 : 0;
+  a.dst_bf16 = (uint16_t*)dst_bf16;
+  a.dst_f32 = dst_f32;
+  hipLaunchKernelGGL(pack_kernel, dim3(nblk(npix * Cp, 256)), dim3(256), 0, (hipStream_t)stream, a);
+  CL_LAUNCHCHK("conv_pack");
+  return 0;
+}
+
+int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src, void* const* dst,
+                    int64_t pstride, int dst_f64, const float* scale) {
+  if (!src || !dst || npix < 1 || C < 1 || C > DDX_CONV_MAXCH || Cp < C || pstride < 1)
+    return ddx::set_error(DDX_ERR_ARG, "bad unpack arguments");
+  UnpackArgs a{};
+  for (int c = 0; c < C; ++c) {
+    if (!dst[c]) return ddx::set_error(DDX_ERR_ARG, "null channel destination");
+    a.dst[c] = dst[c];
+    a.scale[c] = scale ? scale[c] : 1.0f;
+  }
+  a.pstride = pstride;
+  a.npix = npix;
+  a.C = C;
+  a.Cp = Cp;
+  a.f64 = dst_f64 ? 1 : 0;
+  a.src = src;
+  hipLaunchKernelGGL(unpack_kernel, dim3(nblk(npix * C, 256)), dim3(256), 0, (hipStream_t)stream, a);
+  CL_LAUNCHCHK("conv_unpack");
+  return 0;
+}
+
+int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void* src, void* dst) {
+  if (!src || !dst || ny < 1 || nx < 1 || factor < 1 || ny % factor || nx % factor)
+    return ddx::set_error(DDX_ERR_ARG, "box filter: extents must be positive multiples of the factor");
+  const long n = (long)(ny / factor) * (nx / factor);
+  hipStream_t s = (hipStream_t)stream;
+  if (f64)
+    hipLaunchKernelGGL(box_filter_kernel<double>, dim3(nblk(n, 256)), dim3(256), 0, s,
+                       (const double*)src, (double*)dst, ny, nx, factor);
+  else
+    hipLaunchKernelGGL(box_filter_kernel<float>, dim3(nblk(n, 256)), dim3(256), 0, s,
+                       (const float*)src, (float*)dst, ny, nx, factor);
+  CL_LAUNCHCHK("box_filter");
+  return 0;
+}
+
+int ddx_mfma_probe(void* stream, const void* A, const void* Bm, float* D) {
+  if (!A || !Bm || !D) return ddx::set_error(DDX_ERR_ARG, "null argument");
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                     (const uint16_t*)A, (const uint16_t*)Bm, D);
+  CL_LAUNCHCHK("mfma_probe");
+  return 0;
+}
+
+}  // extern "C"

@@ -50,7 +50,8 @@ class Params(ctypes.Structure):
                 ("rho0", ctypes.c_double)]
 
 
-# name -> (restype, argtypes); every exported symbol of include/ddx.h
+# name -> (restype, argtypes); every exported symbol of include/ddx.h and
+# include/ddx_closure.h
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -68,6 +69,7 @@ SIGNATURES = {
     "ddx_plan_kshape": (_I, [_VP, _I64P, _I64P, _I64P, _I64P]),
     "ddx_plan_ktable": (_I, [_VP, _I, _DP, ctypes.c_int64]),
     "ddx_plan_set_stream": (_I, [_VP, _VP]),
+    "ddx_plan_set_graph": (_I, [_VP, _I]),
     "ddx_alloc": (_I, [_SZ, _PP]),
     "ddx_free": (_I, [_VP]),
     "ddx_h2d": (_I, [_VP, _VP, _VP, _SZ]),
@@ -83,14 +85,28 @@ SIGNATURES = {
     "ddx_project": (_I, [_VP, _PP]),
     "ddx_update": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _D, _I, _D]),
     "ddx_rhs": (_I, [_VP, ctypes.POINTER(Params), _PP, _PP, _PP]),
-    "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _I, _D]),
+    "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
     "ddx_kcombine": (_I, [_VP, _VP, _I, _PP, _DP, ctypes.POINTER(_I), _I]),
     "ddx_xcombine": (_I, [_VP, _VP, _I, _PP, _PP, _DP, _I]),
     "ddx_xreduce": (_I, [_VP, _VP, _I, _DP]),
     "ddx_kreduce": (_I, [_VP, _VP, _VP, _I, _DP]),
     "ddx_prof_enable": (_I, [_VP, _I]),
     "ddx_prof_read": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.c_char_p, _I64P, _DP, _DP]),
+    # include/ddx_closure.h
+    "ddx_conv3x3": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+    "ddx_conv3x3_wgrad": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP]),
+    "ddx_conv_colsum": (_I, [_VP, _VP, ctypes.c_int64, _I, _VP]),
+    "ddx_conv_wprep": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _VP]),
+    "ddx_conv_wunpack": (_I, [_VP, _VP, _I, _I, _I, _I, _VP]),
+    "ddx_conv_pack": (_I, [_VP, ctypes.c_int64, _I, _I, _PP, ctypes.c_int64, _I, _VP, _VP, _VP]),
+    "ddx_conv_unpack": (_I, [_VP, ctypes.c_int64, _I, _I, _VP, _PP, ctypes.c_int64, _I, _VP]),
+    "ddx_box_filter": (_I, [_VP, _I, _I, _I, _I, _VP, _VP]),
+    "ddx_mfma_probe": (_I, [_VP, _VP, _VP, _VP]),
 }
+
+# enums of include/ddx_closure.h
+CONV_LINEAR, CONV_GELU, CONV_GELU_GRAD = 0, 1, 2
+CONV_MAXCH = 16
 
 _lib = None
 _lock = threading.Lock()
@@ -221,6 +237,14 @@ class Plan:
     def sync(self):
         _check(_lib.ddx_sync(self.handle))
 
+    def set_graph(self, on):
+        _check(_lib.ddx_plan_set_graph(self.handle, 1 if on else 0))
+
+    def set_stream(self, stream):
+        """Launch on an existing hipStream_t (an int handle); None or 0 restores
+        the plan's own stream."""
+        _check(_lib.ddx_plan_set_stream(self.handle, ctypes.c_void_p(int(stream) if stream else None)))
+
     def copy(self, dst, src, nbytes=None):
         n = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
         _check(_lib.ddx_copy(self.handle, int(dst), int(src), n))
@@ -347,13 +371,16 @@ class Plan:
                             ptr_array([int(s) for s in state]),
                             ptr_array([int(d) for d in deriv]), fz))
 
-    def step(self, integrator, params, state, stage, deriv1, deriv2, nsteps, dt):
+    def step(self, integrator, params, state, stage, deriv1, deriv2, nsteps, dt, forcing=None):
         st = ptr_array([int(s) for s in stage]) if stage is not None else None
+        fz = None
+        if forcing is not None:
+            fz = ptr_array([int(f) if f is not None else None for f in forcing])
         _check(_lib.ddx_step(self.handle, integrator, ctypes.byref(params),
                              ptr_array([int(s) for s in state]), st,
                              ptr_array([int(d) for d in deriv1]),
                              ptr_array([int(d) for d in deriv2]),
-                             int(nsteps), float(dt)))
+                             fz, int(nsteps), float(dt)))
 
 
 def make_params(physics, nu=0., viscosity_order=1, kappa=0., eta=0., g=1., alpha_t=1.,

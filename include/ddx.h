@@ -98,6 +98,11 @@ int ddx_plan_ktable(const ddx_plan* plan, int axis, double* out, int64_t n);
 /* Run subsequent work of this plan on an existing HIP stream (NULL = the
  * plan's own stream). */
 int ddx_plan_set_stream(ddx_plan* plan, void* stream);
+/* ddx_step replays one captured HIP graph per step (on = 1, the default
+ * unless DDX_NO_GRAPH=1) or launches its kernels one by one (on = 0).  The
+ * graph is recaptured whenever the buffers, parameters or dt change.  No
+ * reference counterpart: launch overhead matters on small grids. */
+int ddx_plan_set_graph(ddx_plan* plan, int on);
 
 /* ---- memory --------------------------------------------------------------- */
 
@@ -177,10 +182,13 @@ int ddx_rhs(ddx_plan* plan, const ddx_params* params, const void* const* state,
 /* nsteps RK2mid / RK2trap steps of size dt on compact state in place.
  * stage, deriv1, deriv2 are caller buffers with the state's component
  * count (the reference integrator's data2 / deriv1 / deriv2); stage is
- * unused by RK2trap and may be NULL there. */
+ * unused by RK2trap and may be NULL there.  forcing is as in ddx_rhs and
+ * is added at every right-hand side of the nsteps steps: the reference's
+ * set_velocity_forcing / set_thermal_forcing hook with a function that does
+ * not depend on the state (physics.py:524-525, 569-572). */
 int ddx_step(ddx_plan* plan, int integrator, const ddx_params* params, void* const* state,
-             void* const* stage, void* const* deriv1, void* const* deriv2, int nsteps,
-             double dt);
+             void* const* stage, void* const* deriv1, void* const* deriv2,
+             const void* const* forcing, int nsteps, double dt);
 
 /* ---- instrumentation (no reference counterpart; replaces the wall-clock
  *      Timer of dedalus/utils/timer.py:29-65 at kernel granularity) ------- */

@@ -1,0 +1,95 @@
+/*
+ * ddx_closure.h -- C ABI of the learned sub-grid closure kernels in libddx.so.
+ *
+ * The reference has no learned closure.  Its plug point is the velocity
+ * forcing hook (dedalus/physics/physics.py:524-525, applied at :569-572):
+ * the closure's predicted tendency enters the step as that forcing, through
+ * ddx_rhs / ddx_step (ddx.h).  These entry points are the network's
+ * building blocks: BF16 MFMA 3x3 periodic convolutions with FP32
+ * accumulation and fused epilogues, weight preparation, channel
+ * packing between the solver's x-space fields and the network's
+ * pixel-major activations, and the coarse-graining box filter.
+ *
+ * Conventions
+ *   - Return 0 or a negative DDX_ERR_* code (ddx.h); message in
+ *     ddx_last_error().
+ *   - stream is a hipStream_t (NULL = the null stream); every call is
+ *     asynchronous on it.
+ *   - Activations are pixel-major: [npix][C] with npix = B*H*W in (b, y, x)
+ *     order; bf16 arrays hold raw 16-bit patterns; channel counts Ci, Co of
+ *     the convolutions are padded to multiples of 16 (Co <= 64).
+ *   - Convolution weights are [Co][9][Ci] bf16 with tap t = 3*ky + kx, i.e.
+ *     out[y][x] = sum w[co][3*ky+kx][ci] in[y+ky-1][x+kx-1], indices
+ *     periodic; ddx_conv_wprep builds them from fp32 (co, ci, ky, kx).
+ */
+#ifndef DDX_CLOSURE_H
+#define DDX_CLOSURE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DDX_CONV_LINEAR = 0,    /* out = conv + bias */
+  DDX_CONV_GELU = 1,      /* out_f32 = conv + bias; out_bf16 = GELU(conv + bias) */
+  DDX_CONV_GELU_GRAD = 2  /* out = conv * GELU'(aux): the input gradient through
+                             the previous layer's activation */
+};
+
+enum { DDX_CONV_MAXCH = 16 }; /* channels per pack / unpack call */
+
+/* 3x3 periodic convolution on B images of H x W pixels.  x [npix][Ci] bf16,
+ * w [Co][9][Ci] bf16, bias [Co] fp32 or NULL (ignored in GELU_GRAD mode),
+ * aux [npix][Co] fp32 (GELU_GRAD only).  Writes out_f32 [npix][Co] and/or
+ * out_bf16 [npix][Co] (either may be NULL, not both).  With the transposed
+ * weights of ddx_conv_wprep, the same call gives the input gradient. */
+int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x, const void* w,
+                const float* bias, const float* aux, int mode, float* out_f32, void* out_bf16);
+
+/* Weight gradient dw[co][t][ci] = sum_p dz[p][co] x[shift_t(p)][ci] (fp32,
+ * padded layout).  partial is scratch of nchunk * Co * 9 * Ci floats; the
+ * pixel range is split into at most nchunk chunks whose partial sums are
+ * added in a fixed order (deterministic). */
+int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const void* x,
+                      const void* dz, float* partial, int nchunk, float* dw);
+
+/* out[c] = sum_p a[p][c] (bias gradient), c < C. */
+int ddx_conv_colsum(void* stream, const float* a, int64_t npix, int C, float* out);
+
+/* fp32 weights (co, ci, ky, kx) of Co x Ci -> padded bf16 forward weights
+ * wf [Cop][9][Cip] and transposed weights wt [Cip][9][Cop] with
+ * wt[ci][t][co] = w[co][ci][8 - t] (either output may be NULL). */
+int ddx_conv_wprep(void* stream, const float* w, int Co, int Ci, int Cop, int Cip, void* wf,
+                   void* wt);
+
+/* Padded weight gradient [Cop][9][Cip] -> (co, ci, ky, kx) fp32 of Co x Ci. */
+int ddx_conv_wunpack(void* stream, const float* dwp, int Co, int Ci, int Cop, int Cip, float* dw);
+
+/* Gather C channels into pixel-major padded activations:
+ * dst[p][c] = scale[c] * src[c][p * pstride] for c < C, 0 for C <= c < Cp.
+ * src entries are float64 (src_f64 = 1) or float32; scale may be NULL (1).
+ * Writes dst_bf16 and/or dst_f32. */
+int ddx_conv_pack(void* stream, int64_t npix, int C, int Cp, const void* const* src,
+                  int64_t pstride, int src_f64, const float* scale, void* dst_bf16,
+                  float* dst_f32);
+
+/* Scatter: dst[c][p * pstride] = scale[c] * src[p][c] for c < C (src has Cp
+ * channels per pixel); dst entries float64 (dst_f64 = 1) or float32. */
+int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src,
+                    void* const* dst, int64_t pstride, int dst_f64, const float* scale);
+
+/* Coarse-graining: dst[Y][X] = mean of src over the factor x factor block at
+ * (factor*Y, factor*X); src is ny x nx (float64 if f64, else float32). */
+int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void* src, void* dst);
+
+/* Layout check: D = A B for one 16 x 16 x 16 bf16 tile on one wavefront
+ * (A, B row-major bf16, D row-major fp32). */
+int ddx_mfma_probe(void* stream, const void* A, const void* Bm, float* D);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDX_CLOSURE_H */

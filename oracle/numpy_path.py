@@ -688,9 +688,17 @@ def read_state(data):
     return {name: [c['kspace'].copy() for _, c in f] for name, f in data}
 
 
-def run(kind, shape, kfields, dt, nsteps, integrator='RK2mid', length=None, **params):
-    """Integrate nsteps from kfields and return the final k-space fields."""
+def run(kind, shape, kfields, dt, nsteps, integrator='RK2mid', length=None, forcing=None,
+        **params):
+    """Integrate nsteps from kfields and return the final k-space fields.
+
+    forcing: optional list of reference-layout k arrays, one per velocity
+    component, returned by a state-independent VelocityForcing function
+    (physics.py:524-525, 569-572)."""
     ph = make_physics(kind, shape, length, **params)
+    if forcing is not None:
+        fk = [np.array(f) for f in forcing]
+        ph.forcing_functions['VelocityForcing'] = lambda data, i: fk[i]
     data = load_state(ph, kfields)
     ti = (RK2mid if integrator == 'RK2mid' else RK2trap)(ph)
     for _ in range(nsteps):

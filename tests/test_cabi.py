@@ -1,25 +1,27 @@
-"""The C ABI library loads and exports every symbol include/ddx.h declares (CPU)."""
+"""The C ABI library loads and exports every symbol include/*.h declares (CPU)."""
 
 import ctypes
+import glob
 import os
 import re
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "ddx.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 LIB = os.path.join(ROOT, "dedalus-1.0_amd", "dedalus", "libddx.so")
 
 
 def declared():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(ddx_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_parses():
     names = declared()
-    assert "ddx_init" in names and "ddx_step" in names and len(names) >= 20
+    assert "ddx_init" in names and "ddx_step" in names and "ddx_conv3x3" in names
+    assert len(HEADERS) >= 2 and len(names) >= 30
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libddx.so not built")

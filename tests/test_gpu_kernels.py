@@ -237,3 +237,61 @@ def test_steps(kind, shape, p, integ, dt, nsteps):
     for name in FIELDS[kind]:
         for c in range(len(ref[name])):
             assert rel_l2(got[name][c], ref[name][c]) < 1e-12, (name, c)
+
+
+@pytest.mark.parametrize("kind,shape", [('hydro', (64, 64)), ('mhd', (16, 16, 16))], ids=str)
+def test_graph_replays_match_eager(kind, shape):
+    """Each graph replay, with host work and syncs between calls, equals eager launches bit for bit."""
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("graph replay", kind, shape)
+    kf = kfields(kind, shape, 95, L)
+    prm = params_struct(hp, kind, nd, {'nu': 1e-3, 'eta': 1e-3})
+    runs = []
+    for graph in (False, True):
+        plan = hp.Plan(shape, L)
+        plan.set_graph(graph)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        snaps = []
+        for call in range(5):
+            plan.step(hp.RK2MID, prm, st, bufs[0], bufs[1], bufs[2], 2, 1e-3)
+            snaps.append([plan.download_k(b) for b in st])
+            _ = np.linalg.norm(snaps[-1][0])      # host work between replays
+        runs.append(snaps)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("shape,integ", [((32, 32), 'RK2mid'), ((16, 16, 16), 'RK2trap'),
+                                         ((64, 48), 'RK2mid')], ids=str)
+def test_steps_with_constant_forcing(shape, integ):
+    """ddx_step with a forcing array equals the oracle with a state-independent
+    VelocityForcing function; a second call with other forcing buffers re-keys the graph."""
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("forced steps", shape, integ)
+    kf = kfields('hydro', shape, 50, L)
+    fk = [0.3 * random_k(shape, 60 + i, L) for i in range(nd)]
+    p = {'nu': 2e-3}
+    plan = hp.get_plan(shape, L)
+    prm = params_struct(hp, 'hydro', nd, p)
+    integrator = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+    for scale in (1.0, -0.5):
+        fs = [scale * f for f in fk]
+        ref = npth.run('hydro', shape, kf, 1e-2, 6, integ, L, forcing=fs, **p)
+        st = upload_fields(plan, 'hydro', kf)
+        fb = []
+        for f in fs:
+            b = plan.alloc_k()
+            plan.upload_k(b, f)
+            fb.append(b)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        plan.step(integrator, prm, st, bufs[0], bufs[1], bufs[2], 3, 1e-2, forcing=fb)
+        plan.step(integrator, prm, st, bufs[0], bufs[1], bufs[2], 3, 1e-2, forcing=fb)
+        got = download_fields(plan, 'hydro', nd, st)
+        for c in range(nd):
+            assert rel_l2(got['u'][c], ref['u'][c]) < 1e-12, (scale, c)
