@@ -46,15 +46,25 @@ def _lib():
 
 def conv3x3(x_bf16, w_bf16, B, H, W, bias=None, aux=None, mode=_hip.CONV_LINEAR,
             out_f32=True, out_bf16=False):
-    """One ddx_conv3x3 call: x [npix, Ci] bf16, w [Co, 9, Ci] bf16 -> (f32, bf16) outputs."""
+    """One ddx_conv3x3 call: x [npix, Ci] bf16, w [Co, 9, Ci] bf16 -> (f32, bf16) outputs.
+
+    out_f32 / out_bf16: True (allocate), False (skip) or a preallocated tensor."""
     lib = _lib()
     Ci = x_bf16.shape[1]
     Co = w_bf16.shape[0]
     npix = B * H * W
     assert x_bf16.shape[0] == npix and w_bf16.shape == (Co, 9, Ci)
     assert x_bf16.dtype == torch.bfloat16 and w_bf16.dtype == torch.bfloat16
-    of = torch.empty((npix, Co), dtype=torch.float32, device=x_bf16.device) if out_f32 else None
-    ob = torch.empty((npix, Co), dtype=torch.bfloat16, device=x_bf16.device) if out_bf16 else None
+    if isinstance(out_f32, torch.Tensor):
+        of = out_f32
+        assert of.shape == (npix, Co) and of.dtype == torch.float32 and of.is_contiguous()
+    else:
+        of = torch.empty((npix, Co), dtype=torch.float32, device=x_bf16.device) if out_f32 else None
+    if isinstance(out_bf16, torch.Tensor):
+        ob = out_bf16
+        assert ob.shape == (npix, Co) and ob.dtype == torch.bfloat16 and ob.is_contiguous()
+    else:
+        ob = torch.empty((npix, Co), dtype=torch.bfloat16, device=x_bf16.device) if out_bf16 else None
     if aux is not None:
         assert aux.shape == (npix, Co) and aux.dtype == torch.float32
     _check(lib.ddx_conv3x3(_stream(), B, H, W, Ci, Co, _ptr(x_bf16), _ptr(w_bf16), _ptr(bias),
@@ -104,8 +114,9 @@ def wunpack(dwp, co, ci):
 
 
 def pack(srcs, npix, cp, pstride=1, scale=None, bf16=True, f32=False):
-    """Gather channel sources (tensors or device pointers, all float32 or all
-    float64) into padded pixel-major activations."""
+    """Gather channel sources (tensors, or (device pointer, is_float64) pairs;
+    all of one precision) into padded pixel-major activations.  bf16 / f32:
+    True (allocate), False (skip) or a preallocated [npix, cp] tensor."""
     lib = _lib()
     C = len(srcs)
     ptrs = []
@@ -123,8 +134,16 @@ def pack(srcs, npix, cp, pstride=1, scale=None, bf16=True, f32=False):
             f64 = is64
         assert f64 == is64, "channel sources must share one precision"
     dev = dev or torch.device("cuda", torch.cuda.current_device())
-    ob = torch.empty((npix, cp), dtype=torch.bfloat16, device=dev) if bf16 else None
-    of = torch.empty((npix, cp), dtype=torch.float32, device=dev) if f32 else None
+    if isinstance(bf16, torch.Tensor):
+        ob = bf16
+        assert ob.shape == (npix, cp) and ob.dtype == torch.bfloat16 and ob.is_contiguous()
+    else:
+        ob = torch.empty((npix, cp), dtype=torch.bfloat16, device=dev) if bf16 else None
+    if isinstance(f32, torch.Tensor):
+        of = f32
+        assert of.shape == (npix, cp) and of.dtype == torch.float32 and of.is_contiguous()
+    else:
+        of = torch.empty((npix, cp), dtype=torch.float32, device=dev) if f32 else None
     sc = None
     if scale is not None:
         sc = (ctypes.c_float * C)(*[float(v) for v in scale])
@@ -233,3 +252,52 @@ class ClosureNet(torch.nn.Module):
         """Multiply-adds x 2 per pixel of one forward pass, padded channels."""
         return sum(2 * 9 * pad16(self.chans[l]) * pad16(self.chans[l + 1])
                    for l in range(len(self.chans) - 1))
+
+
+class Inference:
+    """Forward pass of a ClosureNet on fixed-size batches with preallocated
+    buffers: ``run()`` launches only the convolutions (no allocation, no host
+    synchronisation), so it can be captured in a graph.  Call ``refresh()``
+    after the weights change."""
+
+    def __init__(self, net, B, H, W):
+        self.net = net
+        self.shape = (int(B), int(H), int(W))
+        self.npix = B * H * W
+        if self.npix % 16:
+            raise ValueError("B*H*W must be a multiple of 16")
+        dev = net.weights[0].device
+        ch = net.chans
+        self.x = torch.zeros((self.npix, pad16(ch[0])), dtype=torch.bfloat16, device=dev)
+        self.acts = [torch.empty((self.npix, pad16(c)), dtype=torch.bfloat16, device=dev)
+                     for c in ch[1:-1]]
+        self.out = torch.empty((self.npix, pad16(ch[-1])), dtype=torch.float32, device=dev)
+        self.wf = [torch.empty((pad16(ch[l + 1]), 9, pad16(ch[l])), dtype=torch.bfloat16, device=dev)
+                   for l in range(len(ch) - 1)]
+        self.bp = [torch.zeros(pad16(ch[l + 1]), dtype=torch.float32, device=dev)
+                   for l in range(len(ch) - 1)]
+        self.refresh()
+
+    def refresh(self):
+        lib = _lib()
+        ch = self.net.chans
+        for l in range(len(ch) - 1):
+            w = self.net.weights[l].detach().contiguous()
+            _check(lib.ddx_conv_wprep(_stream(), _ptr(w), ch[l + 1], ch[l], pad16(ch[l + 1]),
+                                      pad16(ch[l]), _ptr(self.wf[l]), None))
+            self.bp[l].zero_()
+            self.bp[l][:ch[l + 1]].copy_(self.net.biases[l].detach())
+
+    def run(self):
+        B, H, W = self.shape
+        nl = len(self.net.chans) - 1
+        a = self.x
+        for l in range(nl):
+            if l < nl - 1:
+                conv3x3(a, self.wf[l], B, H, W, bias=self.bp[l], mode=_hip.CONV_GELU,
+                        out_f32=False, out_bf16=self.acts[l])
+                a = self.acts[l]
+            else:
+                conv3x3(a, self.wf[l], B, H, W, bias=self.bp[l], mode=_hip.CONV_LINEAR,
+                        out_f32=self.out, out_bf16=False)
+        return self.out

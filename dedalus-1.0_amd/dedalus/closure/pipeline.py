@@ -1,0 +1,345 @@
+"""Forced 2D turbulence -> (coarse state, sub-grid tendency) pairs -> trained
+closure -> coarse runs judged against the coarse-grained fine run.
+
+Configuration (CLOSURE_512_64):
+  fine    512^2 IncompressibleHydro, FP64, nu = 1e-4, RK2mid, dt = 2.5e-4,
+          state-independent forcing on 3.5 <= |k| < 4.5 with x-space rms 0.1
+          and phases fixed by a seed (the VelocityForcing hook,
+          dedalus/physics/physics.py:524-525, 569-572)
+  coarse  64^2 (8 x 8 box filter), same nu, dt = 2e-3, forcing = B(f)
+  pairs   every 80 fine steps (0.02 time units) after a spin-up of 5 time
+          units, 500 per seed
+  network (u, v, omega) -> tau (2 components), 3x3 periodic convolutions,
+          widths 32, 5 layers, GELU, BF16 MFMA with FP32 accumulation
+
+Time steps: RK2 amplifies an advected mode with y = |u| k dt by
+|G|^2 = 1 + y^4 / 4 per step, which the integrating factor damps by
+exp(-2 nu k^2 dt); growth at the 2/3 cutoff k_c stays below damping for
+dt < (8 nu / (|u|^4 k_c^2))^(1/3): 5.7e-4 at 512^2 and 2.3e-3 at 64^2 for
+|u| = 3.5.  dt = 1e-3 at 512^2 does blow up after about ten time units.
+
+Every launch of a pipeline goes to one torch stream that the plans are bound
+to, so the coarse closure step (features, network, forcing transform, RK2mid
+step) can be captured as one graph.
+"""
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from .. import _hip
+from .net import ClosureNet, Inference, pack, unpack
+from .sgs import CoarseGrainer, band_forcing, initial_velocity, shell_spectrum
+
+CLOSURE_512_64 = dict(
+    fine_shape=(512, 512), factor=8, length=(2 * np.pi, 2 * np.pi), nu=1e-4,
+    dt_fine=2.5e-4, dt_ratio=8, forcing=dict(kmin=3.5, kmax=4.5, rms=0.1), forcing_seed=7,
+    spinup=5.0, sample_every=80, pairs_per_seed=500, train_seeds=(101, 102, 103, 104),
+    eval_seed=201, width=32, depth=5,
+)
+
+
+def small_config(**over):
+    """The same recipe on a smaller grid (tests)."""
+    cfg = dict(CLOSURE_512_64)
+    cfg.update(fine_shape=(128, 128), factor=4, dt_fine=1e-3, dt_ratio=4, spinup=0.5,
+               sample_every=10, pairs_per_seed=16, train_seeds=(11, 12), eval_seed=21,
+               width=16, depth=3)
+    cfg.update(over)
+    return cfg
+
+
+class Pipeline:
+    """Plans, buffers and the stream shared by pair generation and coarse runs."""
+
+    def __init__(self, cfg, stream=None):
+        _hip.init()
+        self.cfg = cfg
+        self.stream = stream or torch.cuda.Stream()
+        self.cg = CoarseGrainer(cfg["fine_shape"], cfg["factor"], cfg["length"],
+                                stream=self.stream.cuda_stream)
+        F, C = self.cg.fine, self.cg.coarse
+        self.coarse_shape = self.cg.coarse_shape
+        self.prm = _hip.make_params(_hip.HYDRO, nu=cfg["nu"])
+        L = cfg["length"]
+        fc = cfg["forcing"]
+        fk = band_forcing(cfg["fine_shape"], L, cfg["forcing_seed"], fc["kmin"], fc["kmax"], fc["rms"])
+        with torch.cuda.stream(self.stream):
+            self.f_fine = [F.alloc_k() for _ in range(2)]
+            for b, a in zip(self.f_fine, fk):
+                F.upload_k(b, a)
+            self.f_coarse = [C.alloc_k() for _ in range(2)]
+            for c in range(2):
+                self.cg.filter_k(self.f_fine[c], self.f_coarse[c])
+            F.sync()
+
+    # ---- fine runs -------------------------------------------------------
+    def fine_state(self, seed):
+        F = self.cg.fine
+        u0 = initial_velocity(self.cfg["fine_shape"], self.cfg["length"], seed)
+        st = [F.alloc_k() for _ in range(2)]
+        with torch.cuda.stream(self.stream):
+            for b, a in zip(st, u0):
+                F.upload_k(b, a)
+        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(3)]
+        return st, bufs
+
+    def fine_advance(self, st, bufs, nsteps):
+        self.cg.fine.step(_hip.RK2MID, self.prm, st, bufs[0], bufs[1], bufs[2], int(nsteps),
+                          self.cfg["dt_fine"], forcing=self.f_fine)
+
+    def max_speed(self, plan, st):
+        """max |u| bound sqrt(max u^2 + max v^2) over x space (host sync)."""
+        x = plan.alloc_x()
+        m = 0.0
+        for b in st:
+            plan.bwd(b, x)
+            m += plan.xreduce(x, 0)
+        return float(np.sqrt(m))
+
+    # ---- pairs -----------------------------------------------------------
+    def generate_pairs(self, seeds=None, log=None):
+        """(X [N, m, m, 3], Y [N, m, m, 2]) float32 on the GPU, N = seeds x pairs_per_seed."""
+        cfg = self.cfg
+        seeds = cfg["train_seeds"] if seeds is None else seeds
+        C = self.cg.coarse
+        m = self.coarse_shape
+        npix = m[0] * m[1]
+        n = len(seeds) * cfg["pairs_per_seed"]
+        dev = torch.device("cuda", torch.cuda.current_device())
+        X = torch.empty((n, m[0], m[1], 3), dtype=torch.float32, device=dev)
+        Y = torch.empty((n, m[0], m[1], 2), dtype=torch.float32, device=dev)
+        ubar = [C.alloc_k() for _ in range(2)]
+        tau = [C.alloc_k() for _ in range(2)]
+        xb = [C.alloc_x() for _ in range(3)]
+        nspin = int(round(cfg["spinup"] / cfg["dt_fine"]))
+        i = 0
+        info = []
+        with torch.cuda.stream(self.stream):
+            for seed in seeds:
+                st, bufs = self.fine_state(seed)
+                self.fine_advance(st, bufs, nspin)
+                umax = []
+                for j in range(cfg["pairs_per_seed"]):
+                    if j:
+                        self.fine_advance(st, bufs, cfg["sample_every"])
+                    self.cg.pair(st, ubar, tau)
+                    self.cg.features_x(ubar, xb)
+                    pack([(int(b), True) for b in xb], npix, 3, bf16=False,
+                         f32=X[i].view(npix, 3))
+                    C.bwd(tau[0], xb[0])
+                    C.bwd(tau[1], xb[1])
+                    pack([(int(xb[0]), True), (int(xb[1]), True)], npix, 2, bf16=False,
+                         f32=Y[i].view(npix, 2))
+                    i += 1
+                    if j % max(1, cfg["pairs_per_seed"] // 4) == 0:
+                        umax.append(self.max_speed(self.cg.fine, st))
+                cfl = max(umax) * cfg["dt_fine"] / (cfg["length"][0] / cfg["fine_shape"][1])
+                info.append(dict(seed=seed, max_speed=max(umax), cfl=cfl))
+                if log:
+                    log("pairs: seed %d done, max |u| %.3f, CFL %.3f" % (seed, max(umax), cfl))
+            torch.cuda.current_stream().synchronize()
+        return X, Y, info
+
+    # ---- coarse runs -----------------------------------------------------
+    def coarse_run(self, net=None, scales=None):
+        return CoarseRun(self, net, scales)
+
+
+def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=None):
+    """Adam on the normalised MSE; returns (scales, history).  The last
+    val_frac of the pairs (the end of the last seed's run) is held out."""
+    n = X.shape[0]
+    nval = max(1, int(round(n * val_frac)))
+    ntr = n - nval
+    xs = X[:ntr].std(dim=(0, 1, 2)).clamp_min(1e-12)
+    ys = Y[:ntr].std(dim=(0, 1, 2)).clamp_min(1e-12)
+    Xn, Yn = X / xs, Y / ys
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=iters, eta_min=lr * 0.02)
+    gen = torch.Generator().manual_seed(seed)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(iters):
+        idx = torch.randint(0, ntr, (batch,), generator=gen).to(X.device)
+        opt.zero_grad(set_to_none=True)
+        loss = ((net(Xn[idx]) - Yn[idx]) ** 2).mean()
+        loss.backward()
+        opt.step()
+        sched.step()
+        if it % 250 == 0 or it == iters - 1:
+            with torch.no_grad():
+                vl = 0.0
+                for s in range(ntr, n, batch):
+                    vl += ((net(Xn[s:s + batch]) - Yn[s:s + batch]) ** 2).sum().item()
+                vl /= nval * Yn[0].numel()
+            hist.append((it, loss.item(), vl))
+            if log:
+                log("train it %d loss %.4f val %.4f (%.1f s)" % (it, loss.item(), vl,
+                                                                   time.perf_counter() - t0))
+    scales = (xs.detach().cpu().numpy().astype(np.float64), ys.detach().cpu().numpy().astype(np.float64))
+    return scales, hist
+
+
+class CoarseRun:
+    """Coarse 2D hydro run on the pipeline's coarse plan, optionally with the
+    learned closure: each step evaluates the network on the step's initial
+    state and adds its tendency (with the coarse-grained forcing) as a
+    constant forcing over the RK2mid step.  ``capture()`` records one step as
+    a graph on the pipeline's stream."""
+
+    def __init__(self, pipe, net=None, scales=None):
+        self.pipe = pipe
+        C = pipe.cg.coarse
+        self.C = C
+        self.dt = pipe.cfg["dt_fine"] * pipe.cfg["dt_ratio"]
+        self.state = [C.alloc_k() for _ in range(2)]
+        self.bufs = [[C.alloc_k() for _ in range(2)] for _ in range(3)]
+        self.net = net
+        self.graph = None
+        m = pipe.coarse_shape
+        self.npix = m[0] * m[1]
+        if net is not None:
+            xs, ys = scales
+            self.in_scale = [1.0 / float(v) for v in xs]
+            self.out_scale = [float(v) for v in ys]
+            with torch.cuda.stream(pipe.stream):
+                self.inf = Inference(net, 1, m[0], m[1])
+            self.xb = [C.alloc_x() for _ in range(3)]
+            self.fx = [C.alloc_x() for _ in range(2)]
+            self.fk = [C.alloc_k() for _ in range(2)]
+
+    def set_state(self, ubar):
+        for c in range(2):
+            self.C.copy(self.state[c], ubar[c])
+
+    def one_step(self):
+        C = self.C
+        forcing = self.pipe.f_coarse
+        if self.net is not None:
+            self.pipe.cg.features_x(self.state, self.xb)
+            pack([(int(b), True) for b in self.xb], self.npix, self.inf.x.shape[1],
+                 scale=self.in_scale, bf16=self.inf.x, f32=False)
+            out = self.inf.run()
+            unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale)
+            for c in range(2):
+                C.fwd(self.fx[c], self.fk[c])
+                C.kcombine(self.fk[c], [(self.pipe.f_coarse[c], 1.0, -1)], accumulate=True)
+            forcing = self.fk
+        C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], self.bufs[2],
+               1, self.dt, forcing=forcing)
+
+    def capture(self):
+        """Record one step as a graph (after one eager warm-up step)."""
+        C = self.C
+        C.set_graph(False)   # the step's own graph would nest inside this capture
+        with torch.cuda.stream(self.pipe.stream):
+            self.one_step()
+            self.pipe.stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self.pipe.stream):
+                self.one_step()
+        self.pipe.stream.synchronize()
+        self.graph = g
+        return g
+
+    def advance(self, nsteps):
+        with torch.cuda.stream(self.pipe.stream):
+            for _ in range(int(nsteps)):
+                if self.graph is not None:
+                    self.graph.replay()
+                else:
+                    self.one_step()
+
+
+def spectrum_of(plan, bufs, shape, length):
+    """Hermitian shell spectrum of device k buffers (host sync)."""
+    return shell_spectrum([plan.download_k(b) for b in bufs], shape, length)[1]
+
+
+def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None, seed=None,
+             log=None):
+    """Run the fine truth and coarse models (with and without closure) from the
+    same coarse-grained state; return spectra and errors.
+
+    Spectra are Hermitian-weighted shell sums (k = 0..kmax of the coarse
+    grid), time-averaged over snapshots t >= avg_from (default T/2).  The
+    error is the mean over 1 <= k <= kc (kc = the coarse 2/3 cutoff) of
+    |E_model(k) / E_truth(k) - 1|."""
+    cfg = pipe.cfg
+    seed = cfg["eval_seed"] if seed is None else seed
+    avg_from = T / 2 if avg_from is None else avg_from
+    F, C = pipe.cg.fine, pipe.cg.coarse
+    L = cfg["length"]
+    cs = pipe.coarse_shape
+    ratio = cfg["dt_ratio"]
+    dtc = cfg["dt_fine"] * ratio
+    nsnap = int(round(T / snap_every))
+    csteps = int(round(snap_every / dtc))
+    nspin = int(round(cfg["spinup"] / cfg["dt_fine"]))
+    ubar = [C.alloc_k() for _ in range(2)]
+    runs = {"none": pipe.coarse_run(None)}
+    if net is not None:
+        runs["closure"] = pipe.coarse_run(net, scales)
+    with torch.cuda.stream(pipe.stream):
+        st, bufs = pipe.fine_state(seed)
+        pipe.fine_advance(st, bufs, nspin)
+        for c in range(2):
+            pipe.cg.filter_k(st[c], ubar[c])
+        for r in runs.values():
+            r.set_state(ubar)
+            r.capture()
+            r.set_state(ubar)
+        spec = {k: [] for k in ["truth"] + list(runs)}
+        times = []
+        for s in range(nsnap):
+            pipe.fine_advance(st, bufs, csteps * ratio)
+            for c in range(2):
+                pipe.cg.filter_k(st[c], ubar[c])
+            for r in runs.values():
+                r.advance(csteps)
+            t = (s + 1) * csteps * dtc
+            times.append(t)
+            spec["truth"].append(spectrum_of(C, ubar, cs, L))
+            for name, r in runs.items():
+                spec[name].append(spectrum_of(C, r.state, cs, L))
+        C.sync()
+    times = np.array(times)
+    sel = times >= avg_from - 1e-12
+    kc = C.cut[0]
+    mean = {k: np.mean(np.array(v)[sel], axis=0) for k, v in spec.items()}
+    ks = np.arange(1, kc + 1)
+    err = {k: float(np.mean(np.abs(mean[k][ks] / mean["truth"][ks] - 1.0)))
+           for k in runs}
+    ens = {k: [float(np.sum(np.arange(len(e)) ** 2 * e)) for e in v] for k, v in spec.items()}
+    res = dict(times=times.tolist(), err=err, kc=int(kc), spectra={k: v.tolist() for k, v in mean.items()},
+               enstrophy=ens, energy={k: [float(e.sum()) for e in v] for k, v in spec.items()})
+    if log:
+        log("evaluate: spectrum error " + ", ".join("%s %.3f" % kv for kv in err.items()))
+    return res
+
+
+def time_steps(pipe, run, nsteps=200):
+    """Mean wall time per replayed coarse step (ms) on the pipeline's stream."""
+    with torch.cuda.stream(pipe.stream):
+        run.advance(5)
+        pipe.stream.synchronize()
+        t0 = time.perf_counter()
+        run.advance(nsteps)
+        pipe.stream.synchronize()
+        return (time.perf_counter() - t0) / nsteps * 1e3
+
+
+def time_fine_steps(pipe, nsteps=200, seed=None):
+    """Mean wall time per fine step (ms): one graph per step, replayed."""
+    seed = pipe.cfg["eval_seed"] if seed is None else seed
+    with torch.cuda.stream(pipe.stream):
+        st, bufs = pipe.fine_state(seed)
+        pipe.fine_advance(st, bufs, 5)
+        pipe.stream.synchronize()
+        t0 = time.perf_counter()
+        pipe.fine_advance(st, bufs, nsteps)
+        pipe.stream.synchronize()
+        return (time.perf_counter() - t0) / nsteps * 1e3
