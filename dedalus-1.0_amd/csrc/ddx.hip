@@ -119,6 +119,11 @@ struct ddx_plan {
   bool generic_only = false;  // DDX_FORCE_GENERIC=1: size-generic kernels everywhere
   // one RK step captured as a HIP graph and replayed (DDX_NO_GRAPH=1 disables)
   bool use_graph = true;
+  // z transforms run (3D); a batched 2D plan keeps members on the z axis
+  // untransformed: compact (Ky, nbatch, Kxp), x space (nbatch, Ny, Nx)
+  bool zfft = false;
+  int nbatch = 1;
+  long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
   std::vector<unsigned char> step_key;
   bool prof = false;
@@ -436,10 +441,10 @@ int fwd_impl(ddx_plan* p, const T* x, cpx<T>* kc) {
   col_defaults(c);
   geom_fy(p, c);
   c.in[0] = A;
-  if (p->ndim == 2) {
+  if (!p->zfft) {
     c.out[0] = kc;
-    c.scale = T(1.0 / (double)p->npts);
-    return launch_col(p, c, 1, 1, -1, false, PC_COL_FWD_Y);
+    c.scale = T(1.0 / (double)p->nper);
+    return launch_col(p, c, p->az.n, 1, -1, false, PC_COL_FWD_Y);
   }
   c.out[0] = B;
   if ((rc = launch_col(p, c, p->az.n, 1, -1, false, PC_COL_FWD_Y))) return rc;
@@ -447,7 +452,7 @@ int fwd_impl(ddx_plan* p, const T* x, cpx<T>* kc) {
   geom_fz(p, c);
   c.in[0] = B;
   c.out[0] = kc;
-  c.scale = T(1.0 / (double)p->npts);
+  c.scale = T(1.0 / (double)p->nper);
   return launch_col(p, c, p->Ky, 1, -1, false, PC_COL_FWD_Z);
 }
 
@@ -459,7 +464,7 @@ int bwd_impl(ddx_plan* p, const cpx<T>* kc, T* x) {
   cpx<T>* B = (cpx<T>*)p->wsB;
   ColArgs<T> c;
   const cpx<T>* ymix;
-  if (p->ndim == 3) {
+  if (p->zfft) {
     col_defaults(c);
     geom_bz(p, c);
     c.in[0] = kc;
@@ -475,7 +480,7 @@ int bwd_impl(ddx_plan* p, const cpx<T>* kc, T* x) {
     geom_by(p, c);
     c.in[0] = kc;
     c.out[0] = B;
-    if ((rc = launch_col(p, c, 1, 1, 1, false, PC_COL_BWD_Y))) return rc;
+    if ((rc = launch_col(p, c, p->az.n, 1, 1, false, PC_COL_BWD_Y))) return rc;
   }
   ymix = B;
   RowArgs<T> r;
@@ -574,7 +579,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int s = 0; s < L.nstate; ++s) c.src[s] = state[s];
   for (int q = 0; q < L.nq; ++q) c.rec[q] = L.rec[q];
   cpx<T>* mixed;  // (Nz, Ny, Kxp) per field
-  if (nd == 3) {
+  if (p->zfft) {
     geom_bz(p, c);
     for (int q = 0; q < L.nq; ++q) c.out[q] = A + q * fs;
     if ((rc = launch_col(p, c, p->Ky, L.nq, 1, true, PC_COL_BWD_Z_SPECTRAL, L.nstate))) return rc;
@@ -585,7 +590,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   } else {
     geom_by(p, c);
     for (int q = 0; q < L.nq; ++q) c.out[q] = B + q * fs;
-    if ((rc = launch_col(p, c, 1, L.nq, 1, true, PC_COL_BWD_Y_SPECTRAL, L.nstate))) return rc;
+    if ((rc = launch_col(p, c, p->az.n, L.nq, 1, true, PC_COL_BWD_Y_SPECTRAL, L.nstate))) return rc;
   }
   mixed = B;
 
@@ -606,7 +611,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   col_defaults(c);
   geom_fy(p, c);
   for (int o = 0; o < L.np; ++o) c.in[o] = A + o * fs;
-  if (nd == 3) {
+  if (p->zfft) {
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
     if ((rc = launch_col(p, c, p->az.n, L.np, -1, false, PC_COL_FWD_Y))) return rc;
     col_defaults(c);
@@ -616,7 +621,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
     Uc = A;
   } else {
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
-    if ((rc = launch_col(p, c, 1, L.np, -1, false, PC_COL_FWD_Y))) return rc;
+    if ((rc = launch_col(p, c, p->az.n, L.np, -1, false, PC_COL_FWD_Y))) return rc;
     Uc = B;
   }
 
@@ -634,7 +639,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   e.physics = prm->physics;
   e.ndim = nd;
   e.bdir = prm->bdir;
-  e.npts = (double)p->npts;
+  e.npts = (double)p->nper;
   e.galpha = prm->g * prm->alpha_t;
   e.beta = prm->beta;
   const int bs = 256;
@@ -823,8 +828,13 @@ int ddx_init(int device) {
   return 0;
 }
 
-int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
-                    ddx_plan** out) {
+}  // extern "C"
+
+namespace {
+
+// nbatch > 0: a 2D plan for nbatch independent members (z axis untransformed)
+int plan_create(int ndim, const int64_t* xshape, const double* length, int precision, int nbatch,
+                ddx_plan** out) {
   if (!out || !xshape || !length) return fail(DDX_ERR_ARG, "null argument");
   *out = nullptr;
   if (g_device < 0) return fail(DDX_ERR_NODEV, "ddx_init has not been called");
@@ -858,11 +868,16 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
       delete p;
       return fail(DDX_ERR_UNSUPPORTED, why);
     }
+  } else if (nbatch > 0) {
+    p->az.n = nbatch; p->az.L = 1.0; p->az.c = nbatch; p->az.K = nbatch;
+    p->az.kfull.assign(nbatch, 0.0);
   } else {
     p->az.n = 1; p->az.L = 1.0; p->az.c = 0; p->az.K = 1; p->az.kfull.assign(1, 0.0);
   }
+  p->zfft = ndim == 3;
+  p->nbatch = nbatch > 0 ? nbatch : 1;
   if (make_fft_plan(p->ax.n, &p->fx) || make_fft_plan(p->ay.n, &p->fy) ||
-      make_fft_plan(p->az.n, &p->fz)) {
+      make_fft_plan(p->zfft ? p->az.n : 1, &p->fz)) {
     delete p;
     return fail(DDX_ERR_UNSUPPORTED, "grid extents must factor into 2, 3 and 5");
   }
@@ -871,8 +886,9 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   p->Ky = p->ay.K;
   p->Kz = p->az.K;
   p->nk = (long)p->Ky * p->Kz * p->Kxp;
-  p->npts = (long)nx * ny * nz;
-  p->nref = (long)(nx / 2 + 1) * ny * nz;
+  p->npts = (long)nx * ny * p->az.n;
+  p->nper = (long)nx * ny * (p->zfft ? p->az.n : 1);
+  p->nref = (long)(nx / 2 + 1) * ny * p->az.n;
 
   // compact k tables
   std::vector<double> hx(p->Kxp, 0.0), hy(p->Ky), hz(p->Kz);
@@ -893,7 +909,7 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   HIPCHK(hipStreamCreate(&p->own_stream));
   p->stream = p->own_stream;
   // FP64 twiddles generated on the device (sincospi); FP32 plans round them
-  const int ns[3] = {p->ax.n, p->ay.n, p->az.n};
+  const int ns[3] = {p->ax.n, p->ay.n, p->zfft ? p->az.n : 1};
   for (int i = 0; i < 3; ++i) {
     const int n = ns[i];
     if (p->tw.count(n)) continue;
@@ -927,6 +943,22 @@ int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int p
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
+                    ddx_plan** out) {
+  return plan_create(ndim, xshape, length, precision, 0, out);
+}
+
+int ddx_plan_create_batched(int nbatch, int64_t ny, int64_t nx, const double* length, int precision,
+                            ddx_plan** out) {
+  if (nbatch < 1 || nbatch > (1 << 20)) return fail(DDX_ERR_ARG, "bad member count");
+  const int64_t xs[2] = {ny, nx};
+  return plan_create(2, xs, length, precision, nbatch, out);
+}
+
 int ddx_plan_destroy(ddx_plan* p) {
   if (!p) return 0;
   if (p->own_stream) (void)hipStreamSynchronize(p->own_stream);
@@ -946,7 +978,7 @@ int ddx_plan_kshape(const ddx_plan* p, int64_t* kref, int64_t* kc, int64_t* cut,
   if (check_plan(p)) return DDX_ERR_ARG;
   const int hx = p->ax.n / 2 + 1;
   if (kref) {
-    if (p->ndim == 2) { kref[0] = hx; kref[1] = p->ay.n; kref[2] = 1; }
+    if (p->ndim == 2) { kref[0] = hx; kref[1] = p->ay.n; kref[2] = p->nbatch; }
     else { kref[0] = p->ay.n; kref[1] = p->az.n; kref[2] = hx; }
   }
   if (kc) { kc[0] = p->Ky; kc[1] = p->Kz; kc[2] = p->Kxp; }

@@ -440,9 +440,11 @@ struct LayoutArgs {
   long nref;            // reference elements
 };
 
+// 2D: fz is the member of a batched plan (0 otherwise); members are stacked
+// (kx, ky) arrays
 __device__ inline long ref_index(const LayoutArgs& L, int m, int fy, int fz) {
   const int hx = L.nx / 2 + 1;
-  if (L.ndim == 2) return (long)m * L.ny + fy;
+  if (L.ndim == 2) return ((long)fz * hx + m) * L.ny + fy;
   return ((long)fy * L.nz + fz) * hx + m;
 }
 
@@ -467,7 +469,13 @@ __global__ void unpack_kernel(const LayoutArgs L, const cpx<T>* kc, cpx<double>*
   if (idx >= L.nref) return;
   const int hx = L.nx / 2 + 1;
   int m, fy, fz;
-  if (L.ndim == 2) { m = (int)(idx / L.ny); fy = (int)(idx % L.ny); fz = 0; }
+  if (L.ndim == 2) {
+    const long per = (long)hx * L.ny;
+    fz = (int)(idx / per);
+    const long r = idx - (long)fz * per;
+    m = (int)(r / L.ny);
+    fy = (int)(r % L.ny);
+  }
   else { m = (int)(idx % hx); long t = idx / hx; fz = (int)(t % L.nz); fy = (int)(t / L.nz); }
   cpx<double> v = mk(0.0, 0.0);
   int row = -1, slab = -1;

@@ -65,6 +65,7 @@ SIGNATURES = {
     "ddx_last_error": (ctypes.c_char_p, []),
     "ddx_version": (_I, []),
     "ddx_plan_create": (_I, [_I, _I64P, _DP, _I, _PP]),
+    "ddx_plan_create_batched": (_I, [_I, ctypes.c_int64, ctypes.c_int64, _DP, _I, _PP]),
     "ddx_plan_destroy": (_I, [_VP]),
     "ddx_plan_kshape": (_I, [_VP, _I64P, _I64P, _I64P, _I64P]),
     "ddx_plan_ktable": (_I, [_VP, _I, _DP, ctypes.c_int64]),
@@ -172,35 +173,50 @@ class DevBuf:
 
 
 class Plan:
-    """One grid: xshape in reference order, box lengths, precision."""
+    """One grid: xshape in reference order, box lengths, precision.
 
-    def __init__(self, shape, length, precision=F64):
+    With ``batch=B`` (2D only) the plan advances B independent members of
+    the grid at once: x arrays are (B, ny, nx), reference k arrays
+    (B, nx/2+1, ny), compact k buffers hold all members."""
+
+    def __init__(self, shape, length, precision=F64, batch=None):
         init()
         self.shape = tuple(int(n) for n in shape)
         self.length = tuple(float(x) for x in length)
         self.ndim = len(self.shape)
         self.precision = precision
-        sh = (ctypes.c_int64 * self.ndim)(*self.shape)
+        self.batch = int(batch) if batch else None
         ln = (ctypes.c_double * self.ndim)(*self.length)
         p = ctypes.c_void_p()
-        _check(_lib.ddx_plan_create(self.ndim, sh, ln, precision, ctypes.byref(p)))
+        if self.batch:
+            if self.ndim != 2:
+                raise ValueError("batched plans are 2D")
+            _check(_lib.ddx_plan_create_batched(self.batch, self.shape[0], self.shape[1], ln,
+                                                precision, ctypes.byref(p)))
+        else:
+            sh = (ctypes.c_int64 * self.ndim)(*self.shape)
+            _check(_lib.ddx_plan_create(self.ndim, sh, ln, precision, ctypes.byref(p)))
         self.handle = p
         kref = (ctypes.c_int64 * 3)()
         kc = (ctypes.c_int64 * 3)()
         cut = (ctypes.c_int64 * 3)()
         nkc = ctypes.c_int64()
         _check(_lib.ddx_plan_kshape(p, kref, kc, cut, ctypes.byref(nkc)))
-        self.kref_shape = tuple(kref[i] for i in range(2 if self.ndim == 2 else 3))
+        if self.batch:
+            self.kref_shape = (kref[2], kref[0], kref[1])
+            self.xshape = (self.batch,) + self.shape
+        else:
+            self.kref_shape = tuple(kref[i] for i in range(2 if self.ndim == 2 else 3))
+            self.xshape = self.shape
         self.kc_shape = tuple(kc[i] for i in range(3))
         self.cut = tuple(cut[i] for i in range(3))
         self.nkc = nkc.value
-        self.npts = int(np.prod(self.shape))
+        self.npts = int(np.prod(self.xshape))
         self.nref = int(np.prod(self.kref_shape))
         self.cbytes = 16 if precision == F64 else 8
         self.rbytes = 8 if precision == F64 else 4
         self.ktables = []
-        for ax, n in ((0, self.cut[0] + 1), (1, 2 * self.cut[1] + 1),
-                      (2, 2 * self.cut[2] + 1)):
+        for ax, n in ((0, self.cut[0] + 1), (1, self.kc_shape[0]), (2, self.kc_shape[1])):
             buf = np.zeros(n, dtype=np.float64)
             _check(_lib.ddx_plan_ktable(p, ax, buf.ctypes.data_as(_DP), n))
             self.ktables.append(buf)
@@ -302,18 +318,18 @@ class Plan:
     def upload_x(self, xd, x):
         """Real x-space array (host) -> device, in the plan's precision."""
         x = np.ascontiguousarray(x, dtype=np.float64 if self.precision == F64 else np.float32)
-        if x.shape != self.shape:
-            raise ValueError("x array shape %s, expected %s" % (x.shape, self.shape))
+        if x.shape != self.xshape:
+            raise ValueError("x array shape %s, expected %s" % (x.shape, self.xshape))
         self.h2d(xd, x)
 
     def download_x(self, xd, out=None):
         """Device x-space array -> float64 host array."""
         if out is None:
-            out = np.empty(self.shape, dtype=np.float64)
+            out = np.empty(self.xshape, dtype=np.float64)
         if self.precision == F64:
             self.d2h(out, xd)
         else:
-            tmp = np.empty(self.shape, dtype=np.float32)
+            tmp = np.empty(self.xshape, dtype=np.float32)
             self.d2h(tmp, xd)
             out[...] = tmp
         return out

@@ -89,9 +89,20 @@ int ddx_version(void);
  * (representations.py:290-316) and _setup_k (representations.py:204-233). */
 int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
                     ddx_plan** plan);
+/* New 2D plan for nbatch independent members on one ny x nx grid (an
+ * ensemble).  No reference counterpart: the reference advances one state
+ * per process; this runs many coarse or ensemble members in one set of
+ * launches.  Per member everything is as for ddx_plan_create(2, {ny, nx}).
+ * Layouts: x space (nbatch, ny, nx); compact k (Ky, nbatch, Kxp), i.e.
+ * the member takes the place of kz; reference k = nbatch stacked (kx, ky)
+ * arrays.  Every call acts on all members at once; ddx_xreduce and
+ * ddx_kreduce reduce over all members together. */
+int ddx_plan_create_batched(int nbatch, int64_t ny, int64_t nx, const double* length,
+                            int precision, ddx_plan** plan);
 int ddx_plan_destroy(ddx_plan* plan);
-/* kref[3] = reference k shape, kc[3] = (Ky, Kz, Kxp), cut[3] = kept |index|
- * per axis (x, y, z); *nkc = Ky*Kz*Kxp. */
+/* kref[3] = reference k shape ((Ny, Nz, Nx/2+1) in 3D, (Nx/2+1, Ny, nbatch)
+ * in 2D with nbatch = 1 for a plain plan), kc[3] = (Ky, Kz, Kxp) with Kz =
+ * nbatch in 2D, cut[3] = kept |index| per axis (x, y, z); *nkc = Ky*Kz*Kxp. */
 int ddx_plan_kshape(const ddx_plan* plan, int64_t* kref, int64_t* kc, int64_t* cut, int64_t* nkc);
 /* Wavenumbers of axis (0 = x, 1 = y, 2 = z) in compact order (n entries). */
 int ddx_plan_ktable(const ddx_plan* plan, int axis, double* out, int64_t n);

@@ -1,0 +1,126 @@
+"""Batched 2D plans (ensembles of independent members) on the GPU.
+
+Every member of a batched plan must equal the same field run through a
+plain 2D plan bit for bit (the kernels are the same; the member index takes
+the place of the z slab), and the plain plan is held to the oracle by the
+kernel tests; a few members are also checked against the oracle directly.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import kfields, random_k, random_x, rel_l2
+from oracle import numpy_path as npth
+
+pytestmark = pytest.mark.gpu
+
+L2 = (2 * np.pi, 2 * np.pi)
+SHAPES = [(32, 32), (64, 64), (48, 32), (128, 128), (256, 256), (512, 512)]
+
+
+def hip():
+    from dedalus import _hip
+    _hip.init()
+    return _hip
+
+
+def case(*args):
+    print("case:", *args, flush=True)
+
+
+def nb_for(shape):
+    return 3 if shape[0] * shape[1] >= 256 * 256 else 7
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=str)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_transforms_match_single_plans(shape, precision):
+    hp = hip()
+    nb = nb_for(shape)
+    prec = hp.F64 if precision == "f64" else hp.F32
+    case("batched transforms", shape, nb, precision)
+    bp = hp.Plan(shape, L2, prec, batch=nb)
+    sp = hp.Plan(shape, L2, prec)
+    xs = np.stack([random_x(shape, 100 + m) for m in range(nb)])
+    ks = np.stack([random_k(shape, 200 + m, L2) for m in range(nb)])
+    xb, kb = bp.alloc_x(), bp.alloc_k()
+    bp.upload_x(xb, xs)
+    bp.fwd(xb, kb)
+    kgot = bp.download_k(kb)
+    bp.upload_k(kb, ks)
+    assert np.array_equal(bp.download_k(kb), np.stack([sp.download_k(_up(sp, k)) for k in ks]))
+    bp.bwd(kb, xb)
+    xgot = bp.download_x(xb)
+    xs1, ks1 = sp.alloc_x(), sp.alloc_k()
+    for m in range(nb):
+        sp.upload_x(xs1, xs[m])
+        sp.fwd(xs1, ks1)
+        assert np.array_equal(kgot[m], sp.download_k(ks1)), m
+        sp.upload_k(ks1, ks[m])
+        sp.bwd(ks1, xs1)
+        assert np.array_equal(xgot[m], sp.download_x(xs1)), m
+
+
+def _up(plan, k):
+    b = plan.alloc_k()
+    plan.upload_k(b, k)
+    return b
+
+
+STEP_CASES = [('hydro', (64, 64), {'nu': 1e-3}), ('hydro', (128, 128), {'nu': 5e-4}),
+              ('boussinesq', (32, 48), {'boussinesq_direction': 'y', 'nu': 1e-3, 'kappa': 1e-3}),
+              ('mhd', (64, 64), {'nu': 1e-3, 'eta': 1e-3}), ('hydro', (256, 256), {'nu': 1e-4})]
+FIELDS = {'hydro': ['u'], 'boussinesq': ['u', 'T'], 'mhd': ['u', 'B']}
+
+
+def params(hp, kind, p):
+    phys = {'hydro': hp.HYDRO, 'boussinesq': hp.BOUSSINESQ, 'mhd': hp.MHD}[kind]
+    return hp.make_params(phys, nu=p.get('nu', 0.), kappa=p.get('kappa', 0.), eta=p.get('eta', 0.),
+                          bdir={'x': 0, 'y': 1}[p.get('boussinesq_direction', 'y')])
+
+
+@pytest.mark.parametrize("kind,shape,p", STEP_CASES, ids=str)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_steps_match_single_plans(kind, shape, p, precision):
+    hp = hip()
+    nb = nb_for(shape)
+    prec = hp.F64 if precision == "f64" else hp.F32
+    case("batched steps", kind, shape, nb, precision)
+    prm = params(hp, kind, p)
+    members = [kfields(kind, shape, 300 + m, L2) for m in range(nb)]
+    comps = [arr for name in FIELDS[kind] for arr in members[0][name]]
+    ncomp = len(comps)
+    stacked = [np.stack([[a for name in FIELDS[kind] for a in mem[name]][c] for mem in members])
+               for c in range(ncomp)]
+    fk = np.stack([0.1 * random_k(shape, 400 + m, L2) for m in range(nb)])
+    bp = hp.Plan(shape, L2, prec, batch=nb)
+    st = [_up(bp, a) for a in stacked]
+    fb = [_up(bp, fk)] + [None] * (ncomp - 1)
+    bufs = [[bp.alloc_k() for _ in st] for _ in range(3)]
+    bp.step(hp.RK2MID, prm, st, bufs[0], bufs[1], bufs[2], 4, 1e-3, forcing=fb)
+    got = [bp.download_k(b) for b in st]
+    sp = hp.Plan(shape, L2, prec)
+    for m in range(nb):
+        s1 = [_up(sp, stacked[c][m]) for c in range(ncomp)]
+        f1 = [_up(sp, fk[m])] + [None] * (ncomp - 1)
+        b1 = [[sp.alloc_k() for _ in s1] for _ in range(3)]
+        sp.step(hp.RK2MID, prm, s1, b1[0], b1[1], b1[2], 4, 1e-3, forcing=f1)
+        for c in range(ncomp):
+            assert np.array_equal(got[c][m], sp.download_k(s1[c])), (m, c)
+
+
+def test_members_match_oracle():
+    hp = hip()
+    shape, nb = (32, 32), 4
+    case("batched members vs oracle", shape, nb)
+    p = {'nu': 2e-3}
+    members = [kfields('hydro', shape, 500 + m, L2) for m in range(nb)]
+    bp = hp.Plan(shape, L2, hp.F64, batch=nb)
+    st = [_up(bp, np.stack([mem['u'][c] for mem in members])) for c in range(2)]
+    bufs = [[bp.alloc_k() for _ in st] for _ in range(3)]
+    bp.step(hp.RK2MID, params(hp, 'hydro', p), st, bufs[0], bufs[1], bufs[2], 5, 1e-2)
+    got = [bp.download_k(b) for b in st]
+    for m in range(nb):
+        ref = npth.run('hydro', shape, members[m], 1e-2, 5, 'RK2mid', L2, **p)
+        for c in range(2):
+            assert rel_l2(got[c][m], ref['u'][c]) < 1e-12, (m, c)
