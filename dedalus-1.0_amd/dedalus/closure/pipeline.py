@@ -31,7 +31,7 @@ import torch
 
 from .. import _hip
 from .net import ClosureNet, Inference, pack, unpack
-from .sgs import CoarseGrainer, band_forcing, initial_velocity, shell_spectrum
+from .sgs import CoarseGrainer, band_forcing, features_x, initial_velocity, shell_spectrum
 
 CLOSURE_512_64 = dict(
     fine_shape=(512, 512), factor=8, length=(2 * np.pi, 2 * np.pi), nu=1e-4,
@@ -144,8 +144,8 @@ class Pipeline:
         return X, Y, info
 
     # ---- coarse runs -----------------------------------------------------
-    def coarse_run(self, net=None, scales=None):
-        return CoarseRun(self, net, scales)
+    def coarse_run(self, net=None, scales=None, batch=None):
+        return CoarseRun(self, net, scales, batch)
 
 
 def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=None):
@@ -184,15 +184,29 @@ def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=No
 
 
 class CoarseRun:
-    """Coarse 2D hydro run on the pipeline's coarse plan, optionally with the
-    learned closure: each step evaluates the network on the step's initial
-    state and adds its tendency (with the coarse-grained forcing) as a
-    constant forcing over the RK2mid step.  ``capture()`` records one step as
-    a graph on the pipeline's stream."""
+    """Coarse 2D hydro run, optionally with the learned closure: each step
+    evaluates the network on the step's initial state and adds its tendency
+    (with the coarse-grained forcing) as a constant forcing over the RK2mid
+    step.  With ``batch=B`` the run advances B members at once on a batched
+    plan (one set of launches for all members, the network on B images).
+    ``capture()`` records one step as a graph on the pipeline's stream."""
 
-    def __init__(self, pipe, net=None, scales=None):
+    def __init__(self, pipe, net=None, scales=None, batch=None):
         self.pipe = pipe
-        C = pipe.cg.coarse
+        self.batch = int(batch) if batch else None
+        if self.batch:
+            C = _hip.Plan(pipe.coarse_shape, pipe.cfg["length"], batch=self.batch)
+            C.set_stream(pipe.stream.cuda_stream)
+            fb = [pipe.cg.coarse.download_k(b) for b in pipe.f_coarse]
+            self.forcing = []
+            with torch.cuda.stream(pipe.stream):
+                for a in fb:
+                    buf = C.alloc_k()
+                    C.upload_k(buf, np.stack([a] * self.batch))
+                    self.forcing.append(buf)
+        else:
+            C = pipe.cg.coarse
+            self.forcing = pipe.f_coarse
         self.C = C
         self.dt = pipe.cfg["dt_fine"] * pipe.cfg["dt_ratio"]
         self.state = [C.alloc_k() for _ in range(2)]
@@ -200,33 +214,42 @@ class CoarseRun:
         self.net = net
         self.graph = None
         m = pipe.coarse_shape
-        self.npix = m[0] * m[1]
+        nb = self.batch or 1
+        self.npix = nb * m[0] * m[1]
         if net is not None:
             xs, ys = scales
             self.in_scale = [1.0 / float(v) for v in xs]
             self.out_scale = [float(v) for v in ys]
             with torch.cuda.stream(pipe.stream):
-                self.inf = Inference(net, 1, m[0], m[1])
+                self.inf = Inference(net, nb, m[0], m[1])
             self.xb = [C.alloc_x() for _ in range(3)]
             self.fx = [C.alloc_x() for _ in range(2)]
             self.fk = [C.alloc_k() for _ in range(2)]
+            self.wk = C.alloc_k()
 
     def set_state(self, ubar):
+        """Copy a coarse velocity (two k buffers of the run's plan, or of the
+        pipeline's single coarse plan for a batched run: every member gets it)."""
+        if self.batch and ubar and ubar[0].nbytes != self.state[0].nbytes:
+            arrs = [self.pipe.cg.coarse.download_k(b) for b in ubar]
+            for c in range(2):
+                self.C.upload_k(self.state[c], np.stack([arrs[c]] * self.batch))
+            return
         for c in range(2):
             self.C.copy(self.state[c], ubar[c])
 
     def one_step(self):
         C = self.C
-        forcing = self.pipe.f_coarse
+        forcing = self.forcing
         if self.net is not None:
-            self.pipe.cg.features_x(self.state, self.xb)
+            features_x(C, self.state, self.xb, self.wk)
             pack([(int(b), True) for b in self.xb], self.npix, self.inf.x.shape[1],
                  scale=self.in_scale, bf16=self.inf.x, f32=False)
             out = self.inf.run()
             unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale)
             for c in range(2):
                 C.fwd(self.fx[c], self.fk[c])
-                C.kcombine(self.fk[c], [(self.pipe.f_coarse[c], 1.0, -1)], accumulate=True)
+                C.kcombine(self.fk[c], [(self.forcing[c], 1.0, -1)], accumulate=True)
             forcing = self.fk
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], self.bufs[2],
                1, self.dt, forcing=forcing)

@@ -90,14 +90,24 @@ class CoarseGrainer:
 
     def vorticity_k(self, ubar, out):
         """omega = d_x v - d_y u (coarse compact k)."""
-        self.coarse.kcombine(out, [(ubar[1], 1.0, 0), (ubar[0], -1.0, 1)])
+        vorticity_k(self.coarse, ubar, out)
 
     def features_x(self, ubar, xbufs):
         """x-space u, v, omega of the coarse velocity into three coarse x buffers."""
-        self.coarse.bwd(ubar[0], xbufs[0])
-        self.coarse.bwd(ubar[1], xbufs[1])
-        self.vorticity_k(ubar, self._wk)
-        self.coarse.bwd(self._wk, xbufs[2])
+        features_x(self.coarse, ubar, xbufs, self._wk)
+
+
+def vorticity_k(plan, u, out):
+    """omega = d_x v - d_y u on any 2D plan (batched plans: every member)."""
+    plan.kcombine(out, [(u[1], 1.0, 0), (u[0], -1.0, 1)])
+
+
+def features_x(plan, u, xbufs, work_k):
+    """x-space u, v, omega of a 2D velocity into three x buffers of the plan."""
+    plan.bwd(u[0], xbufs[0])
+    plan.bwd(u[1], xbufs[1])
+    vorticity_k(plan, u, work_k)
+    plan.bwd(work_k, xbufs[2])
 
 
 def shell_spectrum(kref_comps, shape, length):
