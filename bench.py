@@ -43,6 +43,7 @@ B_ALG_PER_POINT_STEP = {
     "hydro3d_128": 654.4,
     "boussinesq2d_512": 521.3,
     "taylor_green_64": 375.5,
+    "ensemble_768": 370.4,   # 2D hydro, 8 transforms per RHS; halved for its FP32 state
 }
 
 
@@ -114,6 +115,73 @@ def gpu_run(w, steps, warmup, precision="f64"):
     return t1 - t0, prof
 
 
+def ensemble_run(steps, warmup):
+    """The 768-member FP32 ensemble: one graph per step for all three groups."""
+    from dedalus.ensemble import ensemble_768, ENSEMBLE_768
+    ens = ensemble_768()
+    log("ensemble: %d members, %d points" % (ens.nmembers, ens.points))
+    ens.capture()
+    ens.seed_members(ENSEMBLE_768["seed0"])
+    ens.advance(warmup)
+    ens.sync()
+    log("timed %d steps" % steps)
+    t0 = time.perf_counter()
+    ens.advance(steps)
+    ens.sync()
+    t1 = time.perf_counter()
+    log("instrumented %d steps" % steps)
+    for g in ens.groups:
+        g.plan.prof_enable(True)
+    t2 = time.perf_counter()
+    ens.step_eager(steps)
+    ens.sync()
+    t3 = time.perf_counter()
+    prof = {}
+    for g in ens.groups:
+        for k, (n, ms, by) in g.plan.prof_read().items():
+            a = prof.get(k, (0, 0.0, 0.0))
+            prof[k] = (a[0] + n, a[1] + ms, a[2] + by)
+        g.plan.prof_enable(False)
+    import numpy as np
+    e = ens.energies()
+    if not np.all(np.isfinite(e)):
+        raise RuntimeError("non-finite ensemble state after the timed steps")
+    prof["_wall_instrumented_s"] = t3 - t2
+    return t1 - t0, prof, ens
+
+
+def ensemble_cpu_baseline():
+    """One RK2mid step of one member per grid size on the numpy oracle, scaled
+    to the ensemble's point count (FP64 oracle; the GPU state is FP32)."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    from oracle import numpy_path as npth
+    from dedalus.ensemble import ENSEMBLE_768, vorticity_member
+    L = (2 * np.pi, 2 * np.pi)
+    tot_t, tot_pts = 0.0, 0
+    for shape, n in ENSEMBLE_768["groups"]:
+        u0 = vorticity_member(shape, L, ENSEMBLE_768["seed0"])
+        ph = npth.make_physics("hydro", shape, L, nu=ENSEMBLE_768["nu"])
+        data = npth.load_state(ph, {"u": u0})
+        ti = npth.RK2mid(ph)
+        ti.do_advance(data, ENSEMBLE_768["dt"])      # first call sets up transforms
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            ti.do_advance(data, ENSEMBLE_768["dt"])
+        tot_t += (time.perf_counter() - t0) / reps * n
+        tot_pts += shape[0] * shape[1] * n
+    return {
+        "value": tot_pts / tot_t / 1e6,
+        "unit": "Mpt-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "5 RK2mid steps of one member per grid size (64^2, 128^2, 256^2) on the "
+                  "numpy oracle (FP64, single thread), weighted by 256 members each; host %s, "
+                  "%d logical CPUs, numpy %s" % (cpu_model(), os.cpu_count() or 0, np.__version__),
+    }
+
+
 def traffic_from_profiles(kernel, workload):
     """HBM bytes per launch from a committed rocprofv3 summary, if present."""
     path = os.path.join(ROOT, "profiles", "%s_%s_pmc.txt" % (kernel, workload))
@@ -172,12 +240,22 @@ def main():
               "got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from dedalus.init_cond.seeded import workload
-    w = workload(args.workload)
-    npts = 1
-    for n in w["shape"]:
-        npts *= n
-    wall, prof = gpu_run(w, args.steps, args.warmup, args.precision)
+    if args.workload == "ensemble_768":
+        from dedalus.ensemble import ENSEMBLE_768
+        wall, prof, ens = ensemble_run(args.steps, args.warmup)
+        npts = ens.points
+        args.precision = "f32"
+        w = {"name": "ensemble_768", "kind": "hydro",
+             "shape": [list(g[0]) + [g[1]] for g in ENSEMBLE_768["groups"]],
+             "integrator": "RK2mid", "dt": ENSEMBLE_768["dt"],
+             "params": {"nu": ENSEMBLE_768["nu"]}}
+    else:
+        from dedalus.init_cond.seeded import workload
+        w = workload(args.workload)
+        npts = 1
+        for n in w["shape"]:
+            npts *= n
+        wall, prof = gpu_run(w, args.steps, args.warmup, args.precision)
     wall_instr = prof.pop("_wall_instrumented_s")
     if args.calibrate_copy_mib:
         from dedalus import _hip
@@ -228,7 +306,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.precision,
-        "data": "synthetic: seeded initial conditions (dedalus.init_cond.seeded.%s)" % w["name"],
+        "data": ("synthetic: seeded members (dedalus.ensemble.vorticity_member, seed 1000 + m)"
+                 if w["name"] == "ensemble_768" else
+                 "synthetic: seeded initial conditions (dedalus.init_cond.seeded.%s)" % w["name"]),
         "config": {"workload": w["name"], "physics": w["kind"],
                    "grid": list(w["shape"]), "integrator": w["integrator"], "dt": w["dt"],
                    "params": w["params"]},
@@ -237,7 +317,8 @@ def main():
         "kernels": kernels,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w)
+        out["cpu_baseline"] = (ensemble_cpu_baseline() if w["name"] == "ensemble_768"
+                               else cpu_baseline(w))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -680,7 +680,7 @@ int update_impl(ddx_plan* p, int kind, int ncomp, const cpx<T>* const* y, cpx<T>
   const int bs = 256;
   const double kvalid = (double)p->Ky * p->Kz * p->Kx;
   ProfScope prof(p, PC_UPDATE, kvalid * ncomp * (kind >= DDX_ETD2RK1 ? 4 : 3) * sizeof(cpx<T>));
-  hipLaunchKernelGGL((update_kernel<T>), dim3((p->nk + bs - 1) / bs, ncomp), dim3(bs), 0, p->stream, u);
+  hipLaunchKernelGGL((update_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, u);
   LAUNCHCHK();
   return 0;
 }

@@ -356,14 +356,37 @@ __global__ void epilogue_kernel(const EpiArgs<T> a) {
 // phi functions with the reference's branches (forward_step_cy_2d.pyx:123-148,
 // forward_step_cy_3d.pyx:54-59): Taylor series to degree 14 for |Z| < 0.5
 // (phi0 by Taylor in 2D, by exp in 3D), closed forms otherwise.
-__device__ inline double taylor_phi(int k, double z) {
-  double factj = 1.0, sum = 0.0, zp = 1.0;
-  for (int j = k; j < 15; ++j) {
-    if (j > 1) factj *= j;
-    sum += zp / factj;
-    zp *= z;
+// 1 / j! for j = 0..14, evaluated at compile time
+constexpr double inv_fact(int j) {
+  double f = 1.0;
+  for (int i = 2; i <= j; ++i) f *= i;
+  return 1.0 / f;
+}
+
+// phi_0, phi_1, phi_2 of z.  For |z| < 0.5 the reference's Taylor series
+// (the exponential's series truncated after z^14 / 14!, every phi_k from
+// the same terms): phi_2 = sum_{j=2..14} z^(j-2) / j! by Horner, then
+// phi_1 = 1 + z phi_2 and phi_0 = 1 + z phi_1 (identities of the truncated
+// sums), or exp(z) for phi_0 when taylor0 is 0.  Otherwise the closed forms
+// phi_0 = e^z, phi_1 = (phi_0 - 1) / z, phi_2 = (phi_1 - 1) / z.
+__device__ inline void phi012(double z, int taylor0, double& p0, double& p1, double& p2) {
+  if (z < 0.5 && z > -0.5) {
+    constexpr double c[15] = {inv_fact(0),  inv_fact(1),  inv_fact(2),  inv_fact(3),
+                              inv_fact(4),  inv_fact(5),  inv_fact(6),  inv_fact(7),
+                              inv_fact(8),  inv_fact(9),  inv_fact(10), inv_fact(11),
+                              inv_fact(12), inv_fact(13), inv_fact(14)};
+    double s = c[14];
+#pragma unroll
+    for (int j = 13; j >= 2; --j) s = fma(s, z, c[j]);
+    p2 = s;
+    p1 = fma(z, p2, 1.0);
+    p0 = taylor0 ? fma(z, p1, 1.0) : exp(z);
+  } else {
+    p0 = exp(z);
+    const double r = 1.0 / z;
+    p1 = (p0 - 1.0) * r;
+    p2 = (p1 - 1.0) * r;
   }
-  return sum;
 }
 
 template <typename T>
@@ -381,53 +404,51 @@ struct UpdArgs {
 
 enum { UPD_EULER = 0, UPD_ETD1 = 1, UPD_ETD2RK1 = 2, UPD_ETD2RK2 = 3 };
 
+// One thread per compact mode, all components: the phi functions depend on
+// the mode and the component's coefficient only, so components sharing a
+// coefficient (all velocity components; all field components) share them.
 template <typename T>
 __global__ void update_kernel(const UpdArgs<T> a) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int comp = blockIdx.y;
-  if (idx >= a.nk || comp >= a.ncomp) return;
+  if (idx >= a.nk) return;
   int col;
   double kx, ky, kz;
   kvec<double>(a.g, idx, col, kx, ky, kz);
   if (col >= a.g.Kx) {
-    a.out[comp][idx] = mk(T(0), T(0));
+    for (int comp = 0; comp < a.ncomp; ++comp) a.out[comp][idx] = mk(T(0), T(0));
     return;
   }
   const double dt = a.dt;
-  const cpx<double> y = c64(a.y[comp][idx]);
-  const cpx<double> f1 = c64(a.F1[comp][idx]);
-  const cpx<double> f2 = (a.kind >= UPD_ETD2RK1) ? c64(a.F2[comp][idx]) : mk(0.0, 0.0);
-  double Z = 0.0;
-  const double c = a.coeff[comp];
-  if (c != 0.0 && a.kind != UPD_EULER) {
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    double kp;
-    if (a.order == 1) kp = k2;
-    else if (a.order == 2) kp = k2 * k2;
-    else kp = pow(k2, (double)a.order);
-    Z = (-(c * kp)) * dt;
-  }
-  cpx<double> res;
-  if (Z == 0.0) {
-    if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, dt / 2.0);
-    else if (a.kind == UPD_ETD2RK2) res = y + scale(f2, dt);
-    else res = y + scale(f1, dt);
-  } else {
-    double p0, p1, p2;
-    if (Z < 0.5 && Z > -0.5) {
-      p0 = (a.ndim == 2) ? taylor_phi(0, Z) : exp(Z);
-      p1 = taylor_phi(1, Z);
-      p2 = taylor_phi(2, Z);
-    } else {
-      p0 = exp(Z);
-      p1 = (p0 - 1.0) / Z;
-      p2 = (p1 - 1.0) / Z;
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  double kp;
+  if (a.order == 1) kp = k2;
+  else if (a.order == 2) kp = k2 * k2;
+  else kp = pow(k2, (double)a.order);
+  double cprev = 0.0, Z = 0.0, p0 = 1.0, p1 = 1.0, p2 = 0.5;
+  bool have = false;
+  for (int comp = 0; comp < a.ncomp; ++comp) {
+    const cpx<double> y = c64(a.y[comp][idx]);
+    const cpx<double> f1 = c64(a.F1[comp][idx]);
+    const cpx<double> f2 = (a.kind >= UPD_ETD2RK1) ? c64(a.F2[comp][idx]) : mk(0.0, 0.0);
+    const double c = a.coeff[comp];
+    if (!have || c != cprev) {
+      Z = (c != 0.0 && a.kind != UPD_EULER) ? (-(c * kp)) * dt : 0.0;
+      if (Z != 0.0) phi012(Z, a.ndim == 2, p0, p1, p2);
+      cprev = c;
+      have = true;
     }
-    if (a.kind == UPD_ETD1) res = scale(y, p0) + scale(f1, p1 * dt);
-    else if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, p2 * dt);
-    else res = scale(y, p0) + scale(f2 - f1, 2.0 * p2 * dt) + scale(f1, p1 * dt);
+    cpx<double> res;
+    if (Z == 0.0) {
+      if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, dt / 2.0);
+      else if (a.kind == UPD_ETD2RK2) res = y + scale(f2, dt);
+      else res = y + scale(f1, dt);
+    } else {
+      if (a.kind == UPD_ETD1) res = scale(y, p0) + scale(f1, p1 * dt);
+      else if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, p2 * dt);
+      else res = scale(y, p0) + scale(f2 - f1, 2.0 * p2 * dt) + scale(f1, p1 * dt);
+    }
+    a.out[comp][idx] = cT<T>(res);
   }
-  a.out[comp][idx] = cT<T>(res);
 }
 
 // layout conversion: kref (reference transposed half spectrum) <-> compact

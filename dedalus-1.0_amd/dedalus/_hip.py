@@ -115,12 +115,22 @@ _device = None
 
 
 def load_library(path=LIB_PATH):
-    """Load libddx.so and declare its signatures (no device needed)."""
+    """Load libddx.so and declare its signatures (no device needed).
+
+    torch, when installed, is imported first: its wheel carries its own HIP
+    runtime under the same soname (libamdhip64.so.7) as the system one that
+    libddx.so links, and the first one loaded serves the whole process.
+    Loading the system runtime first would leave torch a second HIP/HSA
+    runtime that finds no device."""
     global _lib
     with _lock:
         if _lib is None:
             if not os.path.exists(path):
                 raise DDXError(-5, "libddx.so is not built (%s); run build_ddx.py" % path)
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             lib = ctypes.CDLL(path)
             for name, (res, args) in SIGNATURES.items():
                 fn = getattr(lib, name)

@@ -1,0 +1,162 @@
+"""Ensembles of independent 2D runs on batched plans.
+
+An ensemble is a list of member groups, one per grid size; each group is a
+batched plan (``ddx_plan_create_batched``) whose every call advances all of
+its members, and ``capture()`` records one RK2mid step of every group as a
+single graph, so a step of the whole ensemble (several grid sizes, hundreds
+of members) is one graph launch.
+
+Each member is exactly a dedalus IncompressibleHydro 2D run
+(dedalus/physics/physics.py:527-599 with RK2mid, time_step.py:275-309): the
+batched plan applies the same kernels per member as a plain plan does.
+
+Workload ENSEMBLE_768: 256 members each of 64^2, 128^2 and 256^2, FP32 state
+(FP64 twiddles, projection and ETD arithmetic), nu = 5e-4, dt = 1e-3, 500
+steps; member m (0..767, in group order) starts from seed 1000 + m: a
+random-phase vorticity with shell spectrum ~ k^-3 on 2 <= |k| <= 10,
+psi = omega / k^2, u = d_y psi, v = -d_x psi (init_cond.py:405-408), scaled
+to rms |u| = 1.  dt: RK2's growth 1 + (|u| k dt)^4 / 8 per step at the 2/3
+cutoff k_c = 85 of 256^2 stays below the viscous damping nu k_c^2 dt for
+dt < (8 nu / (|u|^4 k_c^2))^(1/3) = 1.9e-3 at |u| = 3.
+"""
+
+import numpy as np
+
+from . import _hip
+from .data_objects.kgrid import dealias_mask, wavenumbers
+from .init_cond.seeded import hermitian_kx0, mean_square
+
+ENSEMBLE_768 = dict(groups=(((64, 64), 256), ((128, 128), 256), ((256, 256), 256)),
+                    nu=5e-4, dt=1e-3, nsteps=500, seed0=1000, precision="f32")
+
+
+def vorticity_member(shape, length, seed, kmin=2.0, kmax=10.0, slope=-3.0, rms=1.0):
+    """(u, v) reference-layout k arrays of one member (see the module docstring)."""
+    k, _, ksh = wavenumbers(shape, length)
+    kx = np.broadcast_to(k['x'], ksh)
+    ky = np.broadcast_to(k['y'], ksh)
+    k2 = kx ** 2 + ky ** 2
+    kk = np.sqrt(k2)
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(ksh) + 1j * rng.standard_normal(ksh)
+    band = (kk >= kmin) & (kk <= kmax)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        amp = np.where(band, np.sqrt(np.where(band, kk, 1.0) ** slope / (2 * np.pi * np.where(band, kk, 1.0))), 0.0)
+    w = w * amp
+    w[dealias_mask(shape, length)] = 0.0
+    hermitian_kx0(w, shape)
+    psi = np.where(k2 > 0, w / np.where(k2 > 0, k2, 1.0), 0.0)
+    u = 1j * ky * psi
+    v = -1j * kx * psi
+    ms = mean_square([u, v], shape)
+    s = rms / np.sqrt(ms) if ms > 0 else 0.0
+    return [u * s, v * s]
+
+
+class Group:
+    """Members on one grid: a batched plan and its state and work buffers."""
+
+    def __init__(self, shape, nmembers, length, precision, stream=None):
+        self.shape = tuple(shape)
+        self.n = int(nmembers)
+        self.length = tuple(length)
+        self.plan = _hip.Plan(self.shape, self.length, precision, batch=self.n)
+        if stream:
+            self.plan.set_stream(stream)
+        self.state = [self.plan.alloc_k() for _ in range(2)]
+        self.bufs = [[self.plan.alloc_k() for _ in range(2)] for _ in range(3)]
+
+    @property
+    def points(self):
+        return self.n * self.shape[0] * self.shape[1]
+
+    def load(self, members):
+        """members: list of (u, v) reference-layout arrays, one per member."""
+        for c in range(2):
+            self.plan.upload_k(self.state[c], np.stack([m[c] for m in members]))
+
+    def read(self):
+        """[(u, v)] per member (host)."""
+        arrs = [self.plan.download_k(b) for b in self.state]
+        return [(arrs[0][m], arrs[1][m]) for m in range(self.n)]
+
+    def step(self, params, nsteps, dt):
+        self.plan.step(_hip.RK2MID, params, self.state, self.bufs[0], self.bufs[1], self.bufs[2],
+                       int(nsteps), dt)
+
+
+class Ensemble:
+    """Groups of batched members stepped together."""
+
+    def __init__(self, groups, nu, dt, precision="f32", length=(2 * np.pi, 2 * np.pi), stream=None):
+        import torch
+        _hip.init()
+        self.stream = stream or torch.cuda.Stream()
+        prec = _hip.F32 if precision == "f32" else _hip.F64
+        self.groups = [Group(shape, n, length, prec, self.stream.cuda_stream) for shape, n in groups]
+        self.params = _hip.make_params(_hip.HYDRO, nu=nu)
+        self.dt = float(dt)
+        self.length = tuple(length)
+        self.graph = None
+
+    @property
+    def points(self):
+        return sum(g.points for g in self.groups)
+
+    @property
+    def nmembers(self):
+        return sum(g.n for g in self.groups)
+
+    def seed_members(self, seed0):
+        """Member m (in group order) from seed seed0 + m."""
+        m = 0
+        for g in self.groups:
+            g.load([vorticity_member(g.shape, self.length, seed0 + m + j) for j in range(g.n)])
+            m += g.n
+
+    def step_eager(self, nsteps=1):
+        for _ in range(int(nsteps)):
+            for g in self.groups:
+                g.step(self.params, 1, self.dt)
+
+    def capture(self):
+        """One graph holding one RK2mid step of every group."""
+        import torch
+        for g in self.groups:
+            g.plan.set_graph(False)
+        with torch.cuda.stream(self.stream):
+            self.step_eager(1)                       # workspaces, twiddles
+            self.stream.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=self.stream):
+                self.step_eager(1)
+        self.stream.synchronize()
+        self.graph = graph
+        return graph
+
+    def advance(self, nsteps):
+        import torch
+        with torch.cuda.stream(self.stream):
+            for _ in range(int(nsteps)):
+                if self.graph is not None:
+                    self.graph.replay()
+                else:
+                    self.step_eager(1)
+
+    def sync(self):
+        self.stream.synchronize()
+
+    def energies(self):
+        """<|u|^2>/2 per member (host)."""
+        out = []
+        for g in self.groups:
+            for u, v in g.read():
+                out.append(0.5 * mean_square([u, v], g.shape))
+        return np.array(out)
+
+
+def ensemble_768(stream=None):
+    cfg = ENSEMBLE_768
+    ens = Ensemble(cfg["groups"], cfg["nu"], cfg["dt"], cfg["precision"], stream=stream)
+    ens.seed_members(cfg["seed0"])
+    return ens

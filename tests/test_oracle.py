@@ -390,3 +390,28 @@ def test_etd_exact_for_linear_forcing():
     assert abs(out[0] - exact) < 1e-15
     npth.etd2rk2(np.array([y0]), out, np.array([F]), np.array([F]), np.array([c]), dt, 3)
     assert abs(out[0] - exact) < 1e-15
+
+
+def test_ensemble_member_initial_condition():
+    """Seeded ensemble members: solenoidal, Hermitian on kx = 0, energy only on
+    2 <= |k| <= 10, rms |u| = 1, and distinct per seed."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "dedalus-1.0_amd"))
+    from dedalus.ensemble import vorticity_member
+    from dedalus.closure.sgs import shell_spectrum
+    from dedalus.data_objects.kgrid import wavenumbers
+    from dedalus.init_cond.seeded import mean_square
+    L = (2 * np.pi, 2 * np.pi)
+    for shape in ((64, 64), (128, 96)):
+        u = vorticity_member(shape, L, 1000)
+        k, _, ksh = wavenumbers(shape, L)
+        assert abs(k['x'] * u[0] + k['y'] * u[1]).max() < 1e-14
+        assert abs(mean_square(u, shape) - 1.0) < 1e-12
+        _, E = shell_spectrum(u, shape, L)
+        assert set(np.nonzero(E > 1e-30)[0]) <= set(range(2, 11))
+        c = npth.Component(shape, L)
+        c['kspace'] = u[0].copy()
+        x = c['xspace'].copy()
+        assert abs(np.mean(x ** 2) - mean_square([u[0]], shape)) < 1e-12
+        assert not np.allclose(vorticity_member(shape, L, 1001)[0], u[0])
