@@ -22,6 +22,12 @@ Fields of the JSON line besides the driver's contract:
                 point-step (compulsory traffic of the minimal dataflow)
   cpu_baseline  the numpy oracle (a restatement of the reference's numpy
                 path) timed on this host for one step of the same workload
+  closure       (default workload) the metric's second part: the learned
+                sub-grid closure on 512^2 -> 64^2 forced 2D turbulence --
+                spectrum error of coarse runs with and without it against
+                the coarse-grained fine run, throughput of coarse closure
+                runs (1 and batched members) against the plain 512^2 run,
+                and the MFMA rate of its convolutions
 """
 
 import argparse
@@ -182,6 +188,73 @@ def ensemble_cpu_baseline():
     }
 
 
+def closure_summary(batches=(1, 64, 1024), iters=3000, t_eval=10.0):
+    """The learned closure on 512^2 -> 64^2 forced 2D turbulence: pairs from
+    the fine solver, training, spectrum error of coarse runs against the
+    coarse-grained fine run, throughput against the plain 512^2 run, and the
+    MFMA rate of the network's forward convolutions."""
+    import numpy as np
+    import torch
+    from dedalus.closure import ClosureNet, pipeline as P
+    cfg = dict(P.CLOSURE_512_64)
+    t0 = time.perf_counter()
+    pipe = P.Pipeline(cfg)
+    X, Y, info = pipe.generate_pairs()
+    t_pairs = time.perf_counter() - t0
+    log("closure: %d pairs in %.1f s" % (X.shape[0], t_pairs))
+    net = ClosureNet(cin=3, cout=2, width=cfg["width"], depth=cfg["depth"], seed=0).cuda()
+    t1 = time.perf_counter()
+    scales, hist = P.train(net, X, Y, iters=iters)
+    t_train = time.perf_counter() - t1
+    log("closure: trained %d iterations in %.1f s" % (iters, t_train))
+    res = P.evaluate(pipe, net, scales, T=t_eval)
+    log("closure: spectrum error %s" % res["err"])
+    t_fine = P.time_fine_steps(pipe)
+    ratio = cfg["dt_ratio"]
+    thr = {}
+    for b in batches:
+        r = pipe.coarse_run(net, scales, batch=None if b == 1 else b)
+        r.capture()
+        t = P.time_steps(pipe, r, 50 if b > 64 else 200)
+        thr[str(b)] = {"ms_per_step": round(t, 4),
+                       "speedup_vs_plain_512": round(ratio * t_fine * b / t, 2)}
+    # MFMA rate of the forward convolutions on 256 coarse images
+    from dedalus.closure.net import Inference
+    B, (H, W) = 256, pipe.coarse_shape
+    with torch.cuda.stream(pipe.stream):
+        inf = Inference(net, B, H, W)
+        inf.x.normal_()
+        inf.run()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        ev0.record()
+        for _ in range(reps):
+            inf.run()
+        ev1.record()
+        ev1.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    ch = net.chans
+    alg = sum(2 * 9 * ch[l] * ch[l + 1] for l in range(len(ch) - 1)) * B * H * W
+    padded = net.flops_per_pixel() * B * H * W
+    return {
+        "config": "512^2 -> 64^2 (8x8 box), nu 1e-4, dt 2.5e-4 / 2e-3, forcing |k| = 4 rms 0.1",
+        "pairs": int(X.shape[0]), "pairs_s": round(t_pairs, 1),
+        "train_iters": iters, "train_s": round(t_train, 1),
+        "final_loss": round(hist[-1][1], 4), "val_loss": round(hist[-1][2], 4),
+        "spectrum_err": {k: round(v, 4) for k, v in res["err"].items()},
+        "spectrum_err_def": "mean over 1 <= k <= %d of |E_model/E_truth - 1|, shell sums "
+                            "time-averaged over t in [%g, %g]" % (res["kc"], t_eval / 2, t_eval),
+        "fine_ms_per_step": round(t_fine, 4),
+        "throughput": thr,
+        "conv_roofline": {"bound": "mfma", "kernel": "conv3x3 forward (5 layers, 256 images 64^2)",
+                          "achieved": round(alg / (ms * 1e-3) / 1e12, 2),
+                          "peak": 2500.0, "unit": "TFLOP/s",
+                          "frac": round(alg / (ms * 1e-3) / 1e12 / 2500.0, 5),
+                          "mfma_padded_TFLOPs": round(padded / (ms * 1e-3) / 1e12, 2),
+                          "ms_per_forward": round(ms, 4)},
+    }
+
+
 def traffic_from_profiles(kernel, workload):
     """HBM bytes per launch from a committed rocprofv3 summary, if present."""
     path = os.path.join(ROOT, "profiles", "%s_%s_pmc.txt" % (kernel, workload))
@@ -230,6 +303,8 @@ def main():
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64",
                     help="f32: FP32 transforms and products (FP64 twiddles, projection, ETD)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="skip the learned-closure measurement (the metric's second part)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed region, copy this many MiB once with the library's "
                          "copy kernel (a known-size reference for HBM byte counters)")
@@ -316,6 +391,11 @@ def main():
         "step_roofline": step_roof,
         "kernels": kernels,
     }
+    if not args.no_closure and w["name"] == "mhd3d_256":
+        try:
+            out["closure"] = closure_summary()
+        except Exception as e:  # reported, never silently dropped
+            out["closure"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = (ensemble_cpu_baseline() if w["name"] == "ensemble_768"
                                else cpu_baseline(w))

@@ -132,6 +132,99 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs a) {
   }
 }
 
+// LDS-tiled 3x3 convolution for CI, CO in {16, 32}, W % 64 == 0, H % 4 == 0.
+// One block (4 wavefronts) computes a 4-row x 64-column output tile of one
+// image; wavefront w owns output row y0 + w as four tiles of 16 consecutive
+// pixels.  The input halo (6 rows x 66 columns) and the weights are staged
+// in LDS once per block.  Pixel stride CI + 4 and weight-row stride CI + 4
+// (bf16) place the 16 rows of a fragment read in distinct bank pairs.
+__device__ inline bf16x4 lds4(const uint16_t* p) {
+  const uint2 v = *reinterpret_cast<const uint2*>(p);
+  bf16x4 r;
+  r[0] = (short)(v.x & 0xffffu);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffffu);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+template <int CI, int CO>
+__global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a) {
+  constexpr int PS = CI + 4, WS = CI + 4;
+  constexpr int HR = 6, HC = 66;
+  constexpr int NCO = CO / 16, NKC = CI / 16, CH = CI / 4;
+  __shared__ __align__(16) uint16_t halo[HR * HC * PS];
+  __shared__ __align__(16) uint16_t wl[CO * 9 * WS];
+  const int tid = threadIdx.x;
+  const int tiles_x = a.W / 64, tiles_y = a.H / 4;
+  int bid = blockIdx.x;
+  const int tx = bid % tiles_x;
+  bid /= tiles_x;
+  const int ty = bid % tiles_y;
+  const int b = bid / tiles_y;
+  const int x0 = tx * 64, y0 = ty * 4;
+  for (int i = tid; i < CO * 9 * CH; i += 256) {
+    const int row = i / CH, ch = i - row * CH;
+    *reinterpret_cast<uint2*>(&wl[row * WS + ch * 4]) =
+        *reinterpret_cast<const uint2*>(&a.w[(long)row * CI + ch * 4]);
+  }
+  for (int i = tid; i < HR * HC * CH; i += 256) {
+    const int pix = i / CH, ch = i - pix * CH;
+    const int r = pix / HC, c = pix - r * HC;
+    const int yy = wrapi(y0 - 1 + r, a.H), xx = wrapi(x0 - 1 + c, a.W);
+    *reinterpret_cast<uint2*>(&halo[pix * PS + ch * 4]) =
+        *reinterpret_cast<const uint2*>(&a.x[(((long)b * a.H + yy) * a.W + xx) * CI + ch * 4]);
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int m = lane & 15, kq = lane >> 4;
+  f32x4 acc[4][NCO];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < NCO; ++j) acc[q][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3, dx = t % 3;
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      bf16x4 bv[NCO];
+#pragma unroll
+      for (int j = 0; j < NCO; ++j) bv[j] = lds4(&wl[((j * 16 + m) * 9 + t) * WS + kc * 16 + 4 * kq]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const bf16x4 av = lds4(&halo[((wave + dy) * HC + q * 16 + m + dx) * PS + kc * 16 + 4 * kq]);
+#pragma unroll
+        for (int j = 0; j < NCO; ++j)
+          acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[j], acc[q][j], 0, 0, 0);
+      }
+    }
+  }
+  const long prow = ((long)b * a.H + y0 + wave) * a.W + x0;
+#pragma unroll
+  for (int j = 0; j < NCO; ++j) {
+    const int c = j * 16 + m;
+    const float bias = (a.bias && a.mode != DDX_CONV_GELU_GRAD) ? a.bias[c] : 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long o = (prow + q * 16 + 4 * kq + i) * CO + c;
+        float v = acc[q][j][i];
+        float act;
+        if (a.mode == DDX_CONV_GELU_GRAD) {
+          v *= gelu_grad(a.aux[o]);
+          act = v;
+        } else {
+          v += bias;
+          act = a.mode == DDX_CONV_GELU ? gelu(v) : v;
+        }
+        if (a.out_f32) a.out_f32[o] = v;
+        if (a.out_bf16) a.out_bf16[o] = f2bf(act);
+      }
+  }
+}
+
 // Weight-gradient partials over one chunk of pixels, all nine taps per wavefront:
 // partial[chunk][co][t][ci] = sum_{p in chunk} dz[p][co] x[shift_t(p)][ci].
 // grid.x = chunks, grid.y = (Co/16) * (Ci/16); one wavefront per block.
@@ -183,30 +276,162 @@ __global__ void __launch_bounds__(64) conv3x3_wgrad_kernel(const uint16_t* __res
     }
 }
 
-// out[j] = sum over chunks of partial[c][j], in a fixed order (double accumulation)
-__global__ void sum_chunks_kernel(const float* __restrict__ partial, float* __restrict__ out,
-                                  int nchunk, long n) {
-  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double s = 0.0;
-  for (int c = 0; c < nchunk; ++c) s += partial[(long)c * n + j];
-  out[j] = (float)s;
+// LDS-tiled weight gradient for CI, CO in {16, 32}, W % 64 == 0, H % 4 == 0.
+// Block c accumulates over output tiles c, c + gridDim.x, ... (tile = 4 rows
+// x 64 columns of one image, as in the forward kernel) and writes
+// partial[c][co][t][ci].  Per tile, dz (256 pixels x CO) and the input halo
+// (6 x 66 pixels x CI) are staged channel-major, so the MFMA's K dimension
+// (pixels) is contiguous: A[m = co][k = p] from dzT[co][p] (8-byte reads),
+// B[k = p][n = ci] from haloT[ci][row][col].  Wavefront w owns the
+// (co tile, ci tile) pairs w, w + 4, ... with all nine tap accumulators.
+template <int CI, int CO>
+__global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t* __restrict__ x,
+                                                                 const uint16_t* __restrict__ dz,
+                                                                 float* __restrict__ partial,
+                                                                 int B, int H, int W) {
+  constexpr int DS = 260;            // dzT row stride (bf16): 256 pixels + 4
+  constexpr int HRS = 68;            // halo row stride (66 columns + 2)
+  constexpr int HS = 6 * HRS + 2;    // halo channel stride, odd in 4-byte words
+  constexpr int NCT = CI / 16, NPAIR = (CO / 16) * (CI / 16);
+  constexpr int NPW = (NPAIR + 3) / 4;  // pairs per wavefront
+  __shared__ __align__(16) uint16_t dzT[CO * DS];
+  __shared__ __align__(16) uint16_t haloT[CI * HS];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int m = lane & 15, kq = lane >> 4;
+  const int tiles_x = W / 64, tiles_y = H / 4;
+  const long ntiles = (long)B * tiles_y * tiles_x;
+  f32x4 acc[NPW][9];
+#pragma unroll
+  for (int u = 0; u < NPW; ++u)
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[u][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    long rest = tile;
+    const int tx = (int)(rest % tiles_x);
+    rest /= tiles_x;
+    const int ty = (int)(rest % tiles_y);
+    const int b = (int)(rest / tiles_y);
+    const int x0 = tx * 64, y0 = ty * 4;
+    __syncthreads();  // the previous tile's reads are done
+    // dz: 256 pixels x CO, 4 channels per load, stored transposed
+    for (int i = tid; i < 256 * (CO / 4); i += 256) {
+      const int pix = i / (CO / 4), ch = (i - pix * (CO / 4)) * 4;
+      const int r = pix >> 6, c = pix & 63;
+      const uint2 v = *reinterpret_cast<const uint2*>(
+          &dz[(((long)b * H + y0 + r) * W + x0 + c) * CO + ch]);
+      dzT[(ch + 0) * DS + pix] = (uint16_t)(v.x & 0xffffu);
+      dzT[(ch + 1) * DS + pix] = (uint16_t)(v.x >> 16);
+      dzT[(ch + 2) * DS + pix] = (uint16_t)(v.y & 0xffffu);
+      dzT[(ch + 3) * DS + pix] = (uint16_t)(v.y >> 16);
+    }
+    // halo: 6 x 66 pixels x CI, stored transposed
+    for (int i = tid; i < 6 * 66 * (CI / 4); i += 256) {
+      const int pix = i / (CI / 4), ch = (i - pix * (CI / 4)) * 4;
+      const int r = pix / 66, c = pix - r * 66;
+      const int yy = wrapi(y0 - 1 + r, H), xx = wrapi(x0 - 1 + c, W);
+      const uint2 v = *reinterpret_cast<const uint2*>(
+          &x[(((long)b * H + yy) * W + xx) * CI + ch]);
+      const int o = r * HRS + c;
+      haloT[(ch + 0) * HS + o] = (uint16_t)(v.x & 0xffffu);
+      haloT[(ch + 1) * HS + o] = (uint16_t)(v.x >> 16);
+      haloT[(ch + 2) * HS + o] = (uint16_t)(v.y & 0xffffu);
+      haloT[(ch + 3) * HS + o] = (uint16_t)(v.y >> 16);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NPW; ++u) {
+      const int pair = wave + 4 * u;
+      if (pair >= NPAIR) break;
+      const int co0 = (pair / NCT) * 16, ci0 = (pair % NCT) * 16;
+      for (int ks = 0; ks < 16; ++ks) {       // 16 pixels per K step
+        const int row = ks >> 2, col0 = (ks & 3) * 16;
+        const bf16x4 av = lds4(&dzT[(co0 + m) * DS + ks * 16 + 4 * kq]);
+        const uint16_t* hb = &haloT[(ci0 + m) * HS];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+          const uint16_t* hp = hb + (row + t / 3) * HRS + col0 + 4 * kq + t % 3;
+          bf16x4 bv;
+          bv[0] = (short)hp[0];
+          bv[1] = (short)hp[1];
+          bv[2] = (short)hp[2];
+          bv[3] = (short)hp[3];
+          acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[u][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D[row = co (4 kq + i)][col = ci (m)] of each tap
+#pragma unroll
+  for (int u = 0; u < NPW; ++u) {
+    const int pair = wave + 4 * u;
+    if (pair >= NPAIR) break;
+    const int co0 = (pair / NCT) * 16, ci0 = (pair % NCT) * 16;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        partial[(((long)blockIdx.x * CO + co0 + 4 * kq + i) * 9 + t) * CI + ci0 + m] = acc[u][t][i];
+  }
 }
 
-// Column sums of a [npix][C] fp32 array (bias gradient); one block per column
-__global__ void __launch_bounds__(256) colsum_kernel(const float* __restrict__ a,
-                                                     float* __restrict__ out, long npix, int C) {
-  __shared__ double sh[256];
-  const int c = blockIdx.x;
+// out[j] = sum over chunks of partial[c][j] in a fixed order (double):
+// block = 64 consecutive j x 4 chunk groups (group q sums chunks q, q+4, ...),
+// the four group sums are added in order.
+__global__ void __launch_bounds__(256) sum_chunks_kernel(const float* __restrict__ partial,
+                                                         float* __restrict__ out, int nchunk,
+                                                         long n) {
+  __shared__ double sh[4][64];
+  const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const long j = (long)blockIdx.x * 64 + e;
   double s = 0.0;
-  for (long p = threadIdx.x; p < npix; p += 256) s += a[p * C + c];
-  sh[threadIdx.x] = s;
+  if (j < n)
+    for (int c = q; c < nchunk; c += 4) s += partial[(long)c * n + j];
+  sh[q][e] = s;
+  __syncthreads();
+  if (q == 0 && j < n) out[j] = (float)(((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e]);
+}
+
+// Column sums of a [npix][C] fp32 array (bias gradient), two stages with a
+// fixed summation order.  Stage 1: block b sums pixels [b*chunk, (b+1)*chunk)
+// of every column; the 256 threads read rows of C floats contiguously
+// (thread t: column t % C, pixel offset t / C, G = 256 / C pixel groups).
+__global__ void __launch_bounds__(256) colsum_partial_kernel(const float* __restrict__ a,
+                                                             double* __restrict__ partial,
+                                                             long npix, int C, long chunk) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  const int G = 256 / C;
+  const int c = t % C, g = t / C;
+  const long p0 = (long)blockIdx.x * chunk;
+  const long p1 = p0 + chunk < npix ? p0 + chunk : npix;
+  double s = 0.0;
+  if (g < G)
+    for (long p = p0 + g; p < p1; p += G) s += a[p * C + c];
+  sh[t] = s;
+  __syncthreads();
+  if (t < C) {
+    double r = 0.0;
+    for (int k = 0; k < G; ++k) r += sh[k * C + t];
+    partial[(long)blockIdx.x * C + t] = r;
+  }
+}
+
+// Stage 2: out[c] = sum over blocks of partial[b][c]; one block per column,
+// thread t sums blocks t, t+256, ..., then a fixed-order tree
+__global__ void __launch_bounds__(256) colsum_final_kernel(const double* __restrict__ partial,
+                                                           float* __restrict__ out, int nblk,
+                                                           int C) {
+  __shared__ double sh[256];
+  const int c = blockIdx.x, t = threadIdx.x;
+  double s = 0.0;
+  for (int b = t; b < nblk; b += 256) s += partial[(long)b * C + c];
+  sh[t] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    if (t < k) sh[t] += sh[t + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[c] = (float)sh[0];
+  if (t == 0) out[c] = (float)sh[0];
 }
 
 // fp32 weights in (co, ci, ky, kx) order -> padded bf16 forward and transposed
@@ -345,9 +570,19 @@ int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x
     return ddx::set_error(DDX_ERR_ARG, "DDX_CONV_GELU_GRAD needs the pre-activation");
   ConvArgs a{(const uint16_t*)x, (const uint16_t*)w, bias, aux, out_f32, (uint16_t*)out_bf16,
              B, H, W, Ci, Co, mode};
+  hipStream_t s = (hipStream_t)stream;
+  if (W % 64 == 0 && H % 4 == 0 && (Ci == 16 || Ci == 32) && (Co == 16 || Co == 32)) {
+    const dim3 tgrid((unsigned)((long)B * (H / 4) * (W / 64)));
+    const int key = Ci * 100 + Co;
+    if (key == 1616) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 16>), tgrid, dim3(256), 0, s, a);
+    else if (key == 1632) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 32>), tgrid, dim3(256), 0, s, a);
+    else if (key == 3216) hipLaunchKernelGGL((conv3x3_tile_kernel<32, 16>), tgrid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((conv3x3_tile_kernel<32, 32>), tgrid, dim3(256), 0, s, a);
+    CL_LAUNCHCHK("conv3x3 tile");
+    return 0;
+  }
   const long tiles = (long)B * H * W / 16;
   const dim3 grid(nblk(tiles, 4));
-  hipStream_t s = (hipStream_t)stream;
   switch (Co / 16) {
     case 1: hipLaunchKernelGGL(conv3x3_kernel<1>, grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(conv3x3_kernel<2>, grid, dim3(256), 0, s, a); break;
@@ -364,23 +599,53 @@ int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const v
   if (rc) return rc;
   if (!x || !dz || !partial || !dw || nchunk < 1) return ddx::set_error(DDX_ERR_ARG, "null argument");
   const long npix = (long)B * H * W;
+  hipStream_t s = (hipStream_t)stream;
+  if (W % 64 == 0 && H % 4 == 0 && (Ci == 16 || Ci == 32) && (Co == 16 || Co == 32)) {
+    const long ntiles = (long)B * (H / 4) * (W / 64);
+    const int nch = (int)(ntiles < nchunk ? ntiles : nchunk);
+    const int key = Ci * 100 + Co;
+    if (key == 1616)
+      hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<16, 16>), dim3(nch), dim3(256), 0, s,
+                         (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W);
+    else if (key == 1632)
+      hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<16, 32>), dim3(nch), dim3(256), 0, s,
+                         (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W);
+    else if (key == 3216)
+      hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<32, 16>), dim3(nch), dim3(256), 0, s,
+                         (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W);
+    else
+      hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<32, 32>), dim3(nch), dim3(256), 0, s,
+                         (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W);
+    CL_LAUNCHCHK("conv3x3_wgrad tile");
+    const long n = (long)Co * 9 * Ci;
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
+    CL_LAUNCHCHK("conv3x3_wgrad sum");
+    return 0;
+  }
   long pch = (npix + nchunk - 1) / nchunk;
   pch = (pch + 15) / 16 * 16;
   const int nch = (int)((npix + pch - 1) / pch);  // <= nchunk
-  hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(nch, (Co / 16) * (Ci / 16)), dim3(64), 0, s,
                      (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W, Ci, Co, (int)pch);
   CL_LAUNCHCHK("conv3x3_wgrad");
   const long n = (long)Co * 9 * Ci;
-  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, partial, dw, nch, n);
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
   CL_LAUNCHCHK("conv3x3_wgrad sum");
   return 0;
 }
 
-int ddx_conv_colsum(void* stream, const float* a, int64_t npix, int C, float* out) {
-  if (!a || !out || npix < 1 || C < 1) return ddx::set_error(DDX_ERR_ARG, "bad column-sum arguments");
-  hipLaunchKernelGGL(colsum_kernel, dim3(C), dim3(256), 0, (hipStream_t)stream, a, out, (long)npix, C);
+int ddx_conv_colsum(void* stream, const float* a, int64_t npix, int C, double* partial, int nchunk,
+                    float* out) {
+  if (!a || !out || !partial || npix < 1 || C < 1 || C > 256 || nchunk < 1)
+    return ddx::set_error(DDX_ERR_ARG, "bad column-sum arguments");
+  const long chunk = (npix + nchunk - 1) / nchunk;
+  const int nblk = (int)((npix + chunk - 1) / chunk);  // <= nchunk
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(colsum_partial_kernel, dim3(nblk), dim3(256), 0, s, a, partial, (long)npix, C,
+                     chunk);
   CL_LAUNCHCHK("conv_colsum");
+  hipLaunchKernelGGL(colsum_final_kernel, dim3(C), dim3(256), 0, s, partial, out, nblk, C);
+  CL_LAUNCHCHK("conv_colsum final");
   return 0;
 }
 

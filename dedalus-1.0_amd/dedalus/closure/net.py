@@ -87,10 +87,15 @@ def wgrad(x_bf16, dz_bf16, B, H, W, nchunk=None):
     return dw
 
 
-def colsum(a):
+def colsum(a, nchunk=None):
+    """Column sums of a [npix, C] float32 tensor (deterministic)."""
     lib = _lib()
-    out = torch.empty(a.shape[1], dtype=torch.float32, device=a.device)
-    _check(lib.ddx_conv_colsum(_stream(), _ptr(a), a.shape[0], a.shape[1], _ptr(out)))
+    npix, C = a.shape
+    if nchunk is None:
+        nchunk = max(1, min(1024, npix // 1024))
+    partial = torch.empty((nchunk, C), dtype=torch.float64, device=a.device)
+    out = torch.empty(C, dtype=torch.float32, device=a.device)
+    _check(lib.ddx_conv_colsum(_stream(), _ptr(a), npix, C, _ptr(partial), int(nchunk), _ptr(out)))
     return out
 
 

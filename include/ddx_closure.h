@@ -55,8 +55,11 @@ int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x
 int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const void* x,
                       const void* dz, float* partial, int nchunk, float* dw);
 
-/* out[c] = sum_p a[p][c] (bias gradient), c < C. */
-int ddx_conv_colsum(void* stream, const float* a, int64_t npix, int C, float* out);
+/* out[c] = sum_p a[p][c] (bias gradient), c < C <= 256.  partial is scratch
+ * of nchunk * C doubles; pixels are split into at most nchunk chunks summed
+ * in a fixed order (deterministic). */
+int ddx_conv_colsum(void* stream, const float* a, int64_t npix, int C, double* partial,
+                    int nchunk, float* out);
 
 /* fp32 weights (co, ci, ky, kx) of Co x Ci -> padded bf16 forward weights
  * wf [Cop][9][Cip] and transposed weights wt [Cip][9][Cop] with

@@ -61,8 +61,10 @@ def test_mfma_tile_layout():
     assert rl2(D, ref) < 1e-6
 
 
+# shapes with W % 64 == 0 and H % 4 == 0 and 16/32 channels take the LDS-tiled kernel
 CONV_CASES = [(1, 8, 16, 16, 16), (2, 8, 16, 16, 32), (1, 5, 16, 32, 48), (2, 64, 64, 32, 32),
-              (1, 16, 8, 48, 64), (3, 4, 4, 16, 16)]
+              (1, 16, 8, 48, 64), (3, 4, 4, 16, 16), (2, 8, 64, 16, 32), (1, 4, 128, 32, 16),
+              (3, 12, 64, 16, 16), (1, 64, 192, 32, 32)]
 
 
 def random_conv(B, H, W, Ci, Co, seed):
@@ -153,8 +155,12 @@ def test_colsum_pack_unpack():
     case("colsum / pack / unpack")
     g = torch.Generator().manual_seed(6)
     a = torch.randn(4096 + 16, 48, generator=g)
-    s = net.colsum(a.to(d))
-    assert rl2(s, a.double().sum(0)) < 1e-6
+    ad = a.to(d)
+    for nchunk in (None, 1, 7, 4112):
+        s = net.colsum(ad, nchunk)
+        assert rl2(s, a.double().sum(0)) < 1e-6, nchunk
+    b = torch.randn(1 << 16, 32, generator=g)
+    assert rl2(net.colsum(b.to(d)), b.double().sum(0)) < 1e-6
     # planar float64 sources with scales -> padded bf16 + f32
     srcs = [torch.randn(32, 48, dtype=torch.float64, generator=g).to(d) for _ in range(3)]
     scale = [0.5, 2.0, -1.0]
