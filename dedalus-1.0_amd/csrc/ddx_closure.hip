@@ -38,21 +38,27 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
       return ddx::set_error(DDX_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// fp32 -> bf16, round to nearest even; NaN stays a (quiet) NaN
-__device__ inline uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
+// fp32 -> bf16, round to nearest even (gfx950 v_cvt_pk_bf16_f32)
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// GELU with the normal CDF Phi(z) = erfc(-z / sqrt 2) / 2 (erfc keeps the
-// negative tail free of cancellation): GELU(z) = z Phi(z), GELU'(z) = Phi(z) + z phi(z)
-__device__ inline float gelu(float z) { return 0.5f * z * erfcf(-z * 0.70710678118654752f); }
+// GELU in its tanh form, written as z sigma(s) with
+// s = 2 sqrt(2/pi) (z + 0.044715 z^3): 0.5 z (1 + tanh(s/2)) = z sigma(s), so
+// the negative tail has no cancellation.  One hardware exp and one
+// reciprocal per element; sigma(s) = 1 / (1 + e^-s) saturates to 0 and 1.
+//   GELU'(z) = sigma(s) + z sigma(s) (1 - sigma(s)) s'(z),
+//   s'(z) = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2)
+__device__ inline float sigmoid_fast(float s) { return __builtin_amdgcn_rcpf(1.0f + __expf(-s)); }
+__device__ inline float gelu(float z) {
+  const float s = 1.5957691216057308f * (z + 0.044715f * z * z * z);
+  return z * sigmoid_fast(s);
+}
 __device__ inline float gelu_grad(float z) {
-  return 0.5f * erfcf(-z * 0.70710678118654752f) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+  const float s = 1.5957691216057308f * (z + 0.044715f * z * z * z);
+  const float sg = sigmoid_fast(s);
+  const float ds = 1.5957691216057308f * (1.0f + 0.134145f * z * z);
+  return sg + z * sg * (1.0f - sg) * ds;
 }
 
 __device__ inline bf16x4 load4(const uint16_t* p) {
@@ -132,12 +138,6 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(ConvArgs a) {
   }
 }
 
-// LDS-tiled 3x3 convolution for CI, CO in {16, 32}, W % 64 == 0, H % 4 == 0.
-// One block (4 wavefronts) computes a 4-row x 64-column output tile of one
-// image; wavefront w owns output row y0 + w as four tiles of 16 consecutive
-// pixels.  The input halo (6 rows x 66 columns) and the weights are staged
-// in LDS once per block.  Pixel stride CI + 4 and weight-row stride CI + 4
-// (bf16) place the 16 rows of a fragment read in distinct bank pairs.
 __device__ inline bf16x4 lds4(const uint16_t* p) {
   const uint2 v = *reinterpret_cast<const uint2*>(p);
   bf16x4 r;
@@ -148,80 +148,181 @@ __device__ inline bf16x4 lds4(const uint16_t* p) {
   return r;
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// eight consecutive bf16 from LDS as two 8-byte reads
+__device__ inline bf16x8 lds8(const uint16_t* p) {
+  const uint2 lo = *reinterpret_cast<const uint2*>(p);
+  const uint2 hi = *reinterpret_cast<const uint2*>(p + 4);
+  const u32x4 v = {lo.x, lo.y, hi.x, hi.y};
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// LDS-tiled 3x3 convolution for CI, CO in {16, 32}, W % 64 == 0, H % 4 == 0.
+// Output tile = 4 rows x 64 columns of one image; wavefront w owns row
+// y0 + w as four 16-pixel fragments.  Blocks loop over tiles (tile c,
+// c + gridDim.x, ...): the weights are staged in LDS once per block, and the
+// next tile's 6 x 66 input halo is loaded into registers while the current
+// tile computes.  Pixel stride and weight-row stride CI + 4 (bf16) place the
+// 16 rows of a fragment read in distinct bank pairs.  CI = 32 uses the
+// gfx950 16x16x32 bf16 MFMA, CI = 16 the 16x16x16 one.
 template <int CI, int CO>
-__global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a) {
+__global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntiles) {
   constexpr int PS = CI + 4, WS = CI + 4;
   constexpr int HR = 6, HC = 66;
-  constexpr int NCO = CO / 16, NKC = CI / 16, CH = CI / 4;
-  __shared__ __align__(16) uint16_t halo[HR * HC * PS];
+  constexpr int NCO = CO / 16;
+  constexpr int CK = CI / 8;                       // 16-byte chunks per pixel
+  constexpr int NH = HR * HC * CK, HPT = (NH + 255) / 256;
+  constexpr int NW = CO * 9 * CK, WPT = (NW + 255) / 256;
+  // the halo buffer also stages each wavefront's 64 x CO fp32 results
+  constexpr int ES = CO + 4;                                  // staging row stride (floats)
+  constexpr int HALO_B = HR * HC * PS * 2, STAGE_B = 4 * 64 * ES * 4;
+  constexpr int BUF_B = HALO_B > STAGE_B ? HALO_B : STAGE_B;
+  __shared__ __align__(16) uint16_t halo[BUF_B / 2];
   __shared__ __align__(16) uint16_t wl[CO * 9 * WS];
   const int tid = threadIdx.x;
   const int tiles_x = a.W / 64, tiles_y = a.H / 4;
-  int bid = blockIdx.x;
-  const int tx = bid % tiles_x;
-  bid /= tiles_x;
-  const int ty = bid % tiles_y;
-  const int b = bid / tiles_y;
-  const int x0 = tx * 64, y0 = ty * 4;
-  for (int i = tid; i < CO * 9 * CH; i += 256) {
-    const int row = i / CH, ch = i - row * CH;
-    *reinterpret_cast<uint2*>(&wl[row * WS + ch * 4]) =
-        *reinterpret_cast<const uint2*>(&a.w[(long)row * CI + ch * 4]);
-  }
-  for (int i = tid; i < HR * HC * CH; i += 256) {
-    const int pix = i / CH, ch = i - pix * CH;
-    const int r = pix / HC, c = pix - r * HC;
-    const int yy = wrapi(y0 - 1 + r, a.H), xx = wrapi(x0 - 1 + c, a.W);
-    *reinterpret_cast<uint2*>(&halo[pix * PS + ch * 4]) =
-        *reinterpret_cast<const uint2*>(&a.x[(((long)b * a.H + yy) * a.W + xx) * CI + ch * 4]);
-  }
-  __syncthreads();
-  const int wave = tid >> 6, lane = tid & 63;
-  const int m = lane & 15, kq = lane >> 4;
-  f32x4 acc[4][NCO];
+  {
+    uint4 wv[WPT];
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+    for (int k = 0; k < WPT; ++k) {
+      const int i = tid + k * 256;
+      if (i < NW) wv[k] = *reinterpret_cast<const uint4*>(&a.w[(long)i * 8]);
+    }
 #pragma unroll
-    for (int j = 0; j < NCO; ++j) acc[q][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const int dy = t / 3, dx = t % 3;
-#pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) {
-      bf16x4 bv[NCO];
-#pragma unroll
-      for (int j = 0; j < NCO; ++j) bv[j] = lds4(&wl[((j * 16 + m) * 9 + t) * WS + kc * 16 + 4 * kq]);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const bf16x4 av = lds4(&halo[((wave + dy) * HC + q * 16 + m + dx) * PS + kc * 16 + 4 * kq]);
-#pragma unroll
-        for (int j = 0; j < NCO; ++j)
-          acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[j], acc[q][j], 0, 0, 0);
+    for (int k = 0; k < WPT; ++k) {
+      const int i = tid + k * 256;
+      if (i < NW) {
+        const int row = i / CK, ch = i - row * CK;
+        uint2* d = reinterpret_cast<uint2*>(&wl[row * WS + ch * 8]);
+        d[0] = make_uint2(wv[k].x, wv[k].y);
+        d[1] = make_uint2(wv[k].z, wv[k].w);
       }
     }
   }
-  const long prow = ((long)b * a.H + y0 + wave) * a.W + x0;
+  uint4 hv[HPT];
+  auto fetch = [&](long tile) {
+    long rest = tile;
+    const int tx = (int)(rest % tiles_x);
+    rest /= tiles_x;
+    const int ty = (int)(rest % tiles_y);
+    const int b = (int)(rest / tiles_y);
+    const int x0 = tx * 64, y0 = ty * 4;
 #pragma unroll
-  for (int j = 0; j < NCO; ++j) {
-    const int c = j * 16 + m;
-    const float bias = (a.bias && a.mode != DDX_CONV_GELU_GRAD) ? a.bias[c] : 0.f;
+    for (int k = 0; k < HPT; ++k) {
+      const int i = tid + k * 256;
+      if (i < NH) {
+        const int pix = i / CK, ch = i - pix * CK;
+        const int r = pix / HC, c = pix - r * HC;
+        const int yy = wrapi(y0 - 1 + r, a.H), xx = wrapi(x0 - 1 + c, a.W);
+        hv[k] = *reinterpret_cast<const uint4*>(&a.x[(((long)b * a.H + yy) * a.W + xx) * CI + ch * 8]);
+      }
+    }
+  };
+  const int wave = tid >> 6, lane = tid & 63;
+  const int m = lane & 15, kq = lane >> 4;
+  long tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile's fragment reads are done
+#pragma unroll
+    for (int k = 0; k < HPT; ++k) {
+      const int i = tid + k * 256;
+      if (i < NH) {
+        const int pix = i / CK, ch = i - pix * CK;
+        uint2* d = reinterpret_cast<uint2*>(&halo[pix * PS + ch * 8]);
+        d[0] = make_uint2(hv[k].x, hv[k].y);
+        d[1] = make_uint2(hv[k].z, hv[k].w);
+      }
+    }
+    __syncthreads();
+    const long next = tile + gridDim.x;
+    if (next < ntiles) fetch(next);  // in flight while this tile computes
+    f32x4 acc[4][NCO];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long o = (prow + q * 16 + 4 * kq + i) * CO + c;
-        float v = acc[q][j][i];
-        float act;
-        if (a.mode == DDX_CONV_GELU_GRAD) {
-          v *= gelu_grad(a.aux[o]);
-          act = v;
-        } else {
-          v += bias;
-          act = a.mode == DDX_CONV_GELU ? gelu(v) : v;
+      for (int j = 0; j < NCO; ++j) acc[q][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int dy = t / 3, dx = t % 3;
+      if constexpr (CI % 32 == 0) {
+#pragma unroll
+        for (int kc = 0; kc < CI / 32; ++kc) {
+          bf16x8 bv[NCO];
+#pragma unroll
+          for (int j = 0; j < NCO; ++j) bv[j] = lds8(&wl[((j * 16 + m) * 9 + t) * WS + kc * 32 + 8 * kq]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const bf16x8 av = lds8(&halo[((wave + dy) * HC + q * 16 + m + dx) * PS + kc * 32 + 8 * kq]);
+#pragma unroll
+            for (int j = 0; j < NCO; ++j)
+              acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[j], acc[q][j], 0, 0, 0);
+          }
         }
-        if (a.out_f32) a.out_f32[o] = v;
-        if (a.out_bf16) a.out_bf16[o] = f2bf(act);
+      } else {
+#pragma unroll
+        for (int kc = 0; kc < CI / 16; ++kc) {
+          bf16x4 bv[NCO];
+#pragma unroll
+          for (int j = 0; j < NCO; ++j) bv[j] = lds4(&wl[((j * 16 + m) * 9 + t) * WS + kc * 16 + 4 * kq]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const bf16x4 av = lds4(&halo[((wave + dy) * HC + q * 16 + m + dx) * PS + kc * 16 + 4 * kq]);
+#pragma unroll
+            for (int j = 0; j < NCO; ++j)
+              acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv[j], acc[q][j], 0, 0, 0);
+          }
+        }
       }
+    }
+    long rest = tile;
+    const int tx = (int)(rest % tiles_x);
+    rest /= tiles_x;
+    const int ty = (int)(rest % tiles_y);
+    const int b = (int)(rest / tiles_y);
+    const long prow = ((long)b * a.H + ty * 4 + wave) * a.W + tx * 64;
+    // stage D fragments (pixel q*16 + 4kq + i, channel j*16 + m) as [64][ES] fp32
+    __syncthreads();  // every wavefront is done with the halo
+    float* st = reinterpret_cast<float*>(halo) + wave * 64 * ES;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < NCO; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) st[(q * 16 + 4 * kq + i) * ES + j * 16 + m] = acc[q][j][i];
+    __syncthreads();
+    // the wavefront's 64 pixels x CO channels are contiguous in HBM: 4 channels per chunk
+    constexpr int C4 = CO / 4;
+#pragma unroll
+    for (int r = 0; r < (64 * C4) / 64; ++r) {
+      const int id = lane + 64 * r;
+      const int px = id / C4, c = (id - px * C4) * 4;
+      const float4 d = *reinterpret_cast<const float4*>(&st[px * ES + c]);
+      float v[4] = {d.x, d.y, d.z, d.w}, act[4];
+      const long o = (prow + px) * CO + c;
+      if (a.mode == DDX_CONV_GELU_GRAD) {
+        const float4 z = *reinterpret_cast<const float4*>(&a.aux[o]);
+        const float zz[4] = {z.x, z.y, z.z, z.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[e] *= gelu_grad(zz[e]);
+          act[e] = v[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[e] += a.bias ? a.bias[c + e] : 0.f;
+          act[e] = a.mode == DDX_CONV_GELU ? gelu(v[e]) : v[e];
+        }
+      }
+      if (a.out_f32) *reinterpret_cast<float4*>(&a.out_f32[o]) = make_float4(v[0], v[1], v[2], v[3]);
+      if (a.out_bf16)
+        *reinterpret_cast<uint2*>(&a.out_bf16[o]) =
+            make_uint2((uint32_t)f2bf(act[0]) | ((uint32_t)f2bf(act[1]) << 16),
+                       (uint32_t)f2bf(act[2]) | ((uint32_t)f2bf(act[3]) << 16));
+    }
   }
 }
 
@@ -527,6 +628,31 @@ __global__ void box_filter_kernel(const T* __restrict__ src, T* __restrict__ dst
   dst[j] = (T)(s / ((double)f * f));
 }
 
+// Layout probe, K = 32 (gfx950): lane l holds A[l % 16][8 (l / 16) + i] and
+// B[8 (l / 16) + i][l % 16], i < 8; D as for K = 16.
+
+__device__ inline bf16x8 to_bf16x8(const uint16_t (&h)[8]) {
+  bf16x8 r;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r[i] = __builtin_bit_cast(__bf16, h[i]);
+  return r;
+}
+
+__global__ void __launch_bounds__(64) mfma_probe32_kernel(const uint16_t* A, const uint16_t* Bm, float* D) {
+  const int lane = threadIdx.x;
+  const int m = lane & 15, kq = lane >> 4;
+  uint16_t ha[8], hb[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    ha[i] = A[m * 32 + 8 * kq + i];
+    hb[i] = Bm[(8 * kq + i) * 16 + m];
+  }
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(to_bf16x8(ha), to_bf16x8(hb), acc, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) D[(4 * kq + i) * 16 + m] = acc[i];
+}
+
 // Layout probe: D = A B for one 16 x 16 x 16 tile; A, B bf16 row-major, D fp32 row-major.
 __global__ void __launch_bounds__(64) mfma_probe_kernel(const uint16_t* A, const uint16_t* Bm, float* D) {
   const int lane = threadIdx.x;
@@ -571,13 +697,17 @@ int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x
   ConvArgs a{(const uint16_t*)x, (const uint16_t*)w, bias, aux, out_f32, (uint16_t*)out_bf16,
              B, H, W, Ci, Co, mode};
   hipStream_t s = (hipStream_t)stream;
-  if (W % 64 == 0 && H % 4 == 0 && (Ci == 16 || Ci == 32) && (Co == 16 || Co == 32)) {
-    const dim3 tgrid((unsigned)((long)B * (H / 4) * (W / 64)));
+  auto al = [](const void* q, uintptr_t n) { return ((uintptr_t)q % n) == 0; };
+  const bool aligned = al(x, 16) && al(w, 16) && al(out_f32, 16) && al(aux, 16) && al(out_bf16, 8);
+  if (aligned && W % 64 == 0 && H % 4 == 0 && (Ci == 16 || Ci == 32) && (Co == 16 || Co == 32)) {
+    // blocks loop over tiles; about four resident blocks per CU cover the chip
+    const long ntiles = (long)B * (H / 4) * (W / 64);
+    const dim3 tgrid((unsigned)(ntiles < 1024 ? ntiles : 1024));
     const int key = Ci * 100 + Co;
-    if (key == 1616) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 16>), tgrid, dim3(256), 0, s, a);
-    else if (key == 1632) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 32>), tgrid, dim3(256), 0, s, a);
-    else if (key == 3216) hipLaunchKernelGGL((conv3x3_tile_kernel<32, 16>), tgrid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((conv3x3_tile_kernel<32, 32>), tgrid, dim3(256), 0, s, a);
+    if (key == 1616) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 16>), tgrid, dim3(256), 0, s, a, ntiles);
+    else if (key == 1632) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 32>), tgrid, dim3(256), 0, s, a, ntiles);
+    else if (key == 3216) hipLaunchKernelGGL((conv3x3_tile_kernel<32, 16>), tgrid, dim3(256), 0, s, a, ntiles);
+    else hipLaunchKernelGGL((conv3x3_tile_kernel<32, 32>), tgrid, dim3(256), 0, s, a, ntiles);
     CL_LAUNCHCHK("conv3x3 tile");
     return 0;
   }
@@ -727,10 +857,15 @@ int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void
   return 0;
 }
 
-int ddx_mfma_probe(void* stream, const void* A, const void* Bm, float* D) {
+int ddx_mfma_probe(void* stream, int k, const void* A, const void* Bm, float* D) {
   if (!A || !Bm || !D) return ddx::set_error(DDX_ERR_ARG, "null argument");
-  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
-                     (const uint16_t*)A, (const uint16_t*)Bm, D);
+  if (k != 16 && k != 32) return ddx::set_error(DDX_ERR_ARG, "probe K must be 16 or 32");
+  if (k == 16)
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                       (const uint16_t*)A, (const uint16_t*)Bm, D);
+  else
+    hipLaunchKernelGGL(mfma_probe32_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                       (const uint16_t*)A, (const uint16_t*)Bm, D);
   CL_LAUNCHCHK("mfma_probe");
   return 0;
 }

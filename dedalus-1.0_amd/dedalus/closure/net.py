@@ -3,7 +3,7 @@
 Activations are pixel-major [B*H*W, C] (b, y, x order) with C padded to a
 multiple of 16; the network's own input and output are [B, H, W, C] float32
 tensors on the GPU.  Every layer is a 3x3 periodic convolution; hidden
-layers apply GELU (error-function form).  Weights and biases are float32
+layers apply GELU (tanh form, z sigma(2 sqrt(2/pi) (z + 0.044715 z^3))).  Weights and biases are float32
 torch Parameters in torch's (co, ci, ky, kx) layout; each forward pass
 rounds them to bf16 once (``ddx_conv_wprep``) and every product
 accumulates in float32.

@@ -44,7 +44,10 @@ enum { DDX_CONV_MAXCH = 16 }; /* channels per pack / unpack call */
  * w [Co][9][Ci] bf16, bias [Co] fp32 or NULL (ignored in GELU_GRAD mode),
  * aux [npix][Co] fp32 (GELU_GRAD only).  Writes out_f32 [npix][Co] and/or
  * out_bf16 [npix][Co] (either may be NULL, not both).  With the transposed
- * weights of ddx_conv_wprep, the same call gives the input gradient. */
+ * weights of ddx_conv_wprep, the same call gives the input gradient.
+ * Widths W % 64 == 0, heights H % 4 == 0 and 16 or 32 channels with 16-byte
+ * aligned x, w, out_f32, aux (8-byte aligned out_bf16) take the LDS-tiled
+ * kernel; other shapes a per-fragment kernel with the same results. */
 int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x, const void* w,
                 const float* bias, const float* aux, int mode, float* out_f32, void* out_bf16);
 
@@ -87,9 +90,10 @@ int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src,
  * (factor*Y, factor*X); src is ny x nx (float64 if f64, else float32). */
 int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void* src, void* dst);
 
-/* Layout check: D = A B for one 16 x 16 x 16 bf16 tile on one wavefront
- * (A, B row-major bf16, D row-major fp32). */
-int ddx_mfma_probe(void* stream, const void* A, const void* Bm, float* D);
+/* Layout check: D = A B for one 16 x 16 x k bf16 tile (k = 16 or 32) on one
+ * wavefront, with the lane placement the convolution kernels use (A 16 x k,
+ * B k x 16 row-major bf16, D 16 x 16 row-major fp32). */
+int ddx_mfma_probe(void* stream, int k, const void* A, const void* Bm, float* D);
 
 #ifdef __cplusplus
 }

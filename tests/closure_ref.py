@@ -1,7 +1,7 @@
 """CPU reference of the closure network (test infrastructure).
 
 Plain torch float64 on the CPU: periodic 3x3 convolution as circular padding
-+ conv2d, GELU and its derivative from torch autograd, and bf16 rounding at
++ conv2d, GELU (tanh form) and its derivative from torch autograd, and bf16 rounding at
 the points where the GPU path stores bf16 (inputs, weights, hidden
 activations, gradient operands of the MFMA products).  Gradients of the
 convolutions come from torch autograd, independent of the transposed-weight
@@ -31,12 +31,13 @@ def conv_grads(x, w, dz):
 
 
 def gelu(z):
-    return F.gelu(z)
+    """GELU, tanh form (the network's activation)."""
+    return F.gelu(z, approximate="tanh")
 
 
 def gelu_grad(z):
     zr = z.clone().requires_grad_(True)
-    F.gelu(zr).sum().backward()
+    F.gelu(zr, approximate="tanh").sum().backward()
     return zr.grad
 
 
