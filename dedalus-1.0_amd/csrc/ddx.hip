@@ -122,6 +122,9 @@ struct ddx_plan {
   // z transforms run (3D); a batched 2D plan keeps members on the z axis
   // untransformed: compact (Ky, nbatch, Kxp), x space (nbatch, Ny, Nx)
   bool zfft = false;
+  // ddx_step applies each ETD sweep inside the preceding RHS epilogue
+  // (DDX_NO_FUSED_UPDATE=1 runs them as separate launches)
+  bool fuse_update = true;
   int nbatch = 1;
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
@@ -557,9 +560,22 @@ int nstate_of(const ddx_params* prm, int nd) {
   return L.nstate;
 }
 
+// An ETD sweep to fuse into the epilogue of a right-hand side
+template <typename T>
+struct FusedUpdate {
+  int kind;                  // DDX_ETD1 / DDX_ETD2RK1 / DDX_ETD2RK2
+  const cpx<T>* const* y;
+  const cpx<T>* const* f1;   // ETD2RK*: the first stage's derivative
+  cpx<T>* const* out;
+  const double* coeff;
+  int order;
+  double dt;
+  bool write_deriv;
+};
+
 template <typename T>
 int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx<T>* const* deriv,
-             const cpx<T>* const* forcing) {
+             const cpx<T>* const* forcing, const FusedUpdate<T>* fu = nullptr) {
   const int nd = p->ndim;
   PhysLayout L;
   if (phys_layout(prm->physics, nd, &L)) return fail(DDX_ERR_ARG, "unknown physics");
@@ -642,12 +658,29 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   e.npts = (double)p->nper;
   e.galpha = prm->g * prm->alpha_t;
   e.beta = prm->beta;
+  e.upd_kind = -1;
+  int upd_arrays = 0;
+  if (fu) {
+    e.upd_kind = fu->kind;
+    e.upd_order = fu->order;
+    e.upd_dt = fu->dt;
+    e.write_deriv = fu->write_deriv ? 1 : 0;
+    for (int s = 0; s < L.nstate; ++s) {
+      e.upd_y[s] = fu->y[s];
+      e.upd_f1[s] = fu->f1 ? fu->f1[s] : nullptr;
+      e.upd_out[s] = fu->out[s];
+      e.upd_coeff[s] = fu->coeff[s];
+    }
+    // y read, out written, F1 read (ETD2RK*), deriv not written (stage 2)
+    upd_arrays = 2 + (fu->kind >= DDX_ETD2RK1 ? 1 : 0) - (fu->write_deriv ? 0 : 1);
+  }
   const int bs = 256;
   int nforce = 0;
   for (int s = 0; s < L.nstate; ++s) nforce += e.forcing[s] ? 1 : 0;
   const double kvalid = (double)p->Ky * p->Kz * p->Kx;
   const int extra = prm->physics == DDX_BOUSSINESQ ? 2 : 0;
-  ProfScope prof(p, PC_EPILOGUE, kvalid * (L.np + L.nstate + extra + nforce) * sizeof(cpx<T>));
+  ProfScope prof(p, PC_EPILOGUE,
+                 kvalid * (L.np + L.nstate + extra + nforce + upd_arrays * L.nstate) * sizeof(cpx<T>));
   hipLaunchKernelGGL((epilogue_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
   LAUNCHCHK();
   return 0;
@@ -710,6 +743,24 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
   const int vo = prm->viscosity_order;
   auto one_step = [&]() -> int {
     int rc;
+    if (p->fuse_update) {
+      // the same two RHS + two ETD sweeps, each sweep inside the preceding
+      // epilogue; deriv2 is not written (nothing reads it after the step)
+      const cpx<T>* const* yc = (const cpx<T>* const*)y;
+      const cpx<T>* const* d1c = (const cpx<T>* const*)d1;
+      if (integrator == DDX_RK2MID) {
+        // time_step.py:285-305
+        FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, stage, coeff, vo, dt / 2.0, true};
+        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1))) return rc;
+        FusedUpdate<T> u2{DDX_ETD2RK2, yc, d1c, y, coeff, vo, dt, false};
+        return rhs_impl<T>(p, prm, stage, d2, forcing, &u2);
+      }
+      // time_step.py:369-389
+      FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, y, coeff, vo, dt, true};
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1))) return rc;
+      FusedUpdate<T> u2{DDX_ETD2RK1, yc, d1c, y, coeff, vo, dt, false};
+      return rhs_impl<T>(p, prm, y, d2, forcing, &u2);
+    }
     if (integrator == DDX_RK2MID) {
       // time_step.py:285-305
       if ((rc = rhs_impl<T>(p, prm, y, d1, forcing))) return rc;
@@ -853,6 +904,8 @@ int plan_create(int ndim, const int64_t* xshape, const double* length, int preci
     p->generic_only = env && env[0] == '1';
     const char* ng = getenv("DDX_NO_GRAPH");
     p->use_graph = !(ng && ng[0] == '1');
+    const char* nf = getenv("DDX_NO_FUSED_UPDATE");
+    p->fuse_update = !(nf && nf[0] == '1');
   }
   p->ndim = ndim;
   p->precision = precision;

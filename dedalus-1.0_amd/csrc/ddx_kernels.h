@@ -279,6 +279,15 @@ struct EpiArgs {
   double npts;    // N = prod(xshape); the epilogue multiplies by 1 / N
   double galpha;  // g * alpha_t
   double beta;
+  // fused ETD sweep (upd_kind >= 0): upd_out = ETD(upd_y, F1, F2) with
+  // F1 = this right-hand side (ETD1, Euler) or upd_f1 (ETD2RK1/2) and F2 =
+  // this right-hand side; deriv is written only when write_deriv is set
+  int upd_kind, upd_order, write_deriv;
+  double upd_dt;
+  double upd_coeff[DDX_MAXF];
+  const cpx<T>* upd_y[DDX_MAXF];
+  const cpx<T>* upd_f1[DDX_MAXF];
+  cpx<T>* upd_out[DDX_MAXF];
 };
 
 template <typename T>
@@ -299,60 +308,6 @@ __device__ inline cpx<T> cT(cpx<double> v) { return mk(T(v.x), T(v.y)); }
 
 // k-space epilogue of the RHS, computed in FP64 whatever the storage type
 // (projection and curl in double precision).
-template <typename T>
-__global__ void epilogue_kernel(const EpiArgs<T> a) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.nk) return;
-  int col;
-  double kx, ky, kz;
-  kvec<double>(a.g, idx, col, kx, ky, kz);
-  const int nd = a.ndim;
-  const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
-  if (col >= a.g.Kx) {
-    for (int v = 0; v < nvar; ++v) a.deriv[v][idx] = mk(T(0), T(0));
-    return;
-  }
-  const double s = 1.0 / a.npts;
-  const double kk[3] = {kx, ky, kz};
-  cpx<double> F[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    F[i] = mk(0.0, 0.0);
-    if (i < nd) {
-      F[i] = scale(c64(a.U[i][idx]), s);
-      if (a.forcing[i]) F[i] = F[i] + c64(a.forcing[i][idx]);
-      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(c64(a.state[nd][idx]), a.galpha);
-    }
-  }
-  // projection F <- F - k (k.F) / |k|^2, |k|^2 = 0 replaced by 1
-  double k2 = kx * kx + ky * ky + kz * kz;
-  if (k2 == 0.0) k2 = 1.0;
-  cpx<double> kf = mk(0.0, 0.0);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) kf = kf + scale(F[i], kk[i]);
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    if (i < nd) a.deriv[i][idx] = cT<T>(F[i] - scale(kf, kk[i] / k2));
-  if (a.physics == PH_BOUSS) {
-    cpx<double> ft = scale(c64(a.U[nd][idx]), s) - scale(c64(a.state[a.bdir][idx]), a.beta);
-    if (a.forcing[nd]) ft = ft + c64(a.forcing[nd][idx]);
-    a.deriv[nd][idx] = cT<T>(ft);
-  } else if (a.physics == PH_MHD) {
-    if (nd == 3) {
-      const cpx<double> e0 = scale(c64(a.U[3][idx]), s), e1 = scale(c64(a.U[4][idx]), s),
-                        e2 = scale(c64(a.U[5][idx]), s);
-      // i k x E
-      a.deriv[3][idx] = cT<T>(mul_is(e2, ky) - mul_is(e1, kz));
-      a.deriv[4][idx] = cT<T>(mul_is(e0, kz) - mul_is(e2, kx));
-      a.deriv[5][idx] = cT<T>(mul_is(e1, kx) - mul_is(e0, ky));
-    } else {
-      const cpx<double> e = scale(c64(a.U[2][idx]), s);
-      a.deriv[2][idx] = cT<T>(mul_is(e, ky));
-      a.deriv[3][idx] = cT<T>(mk(0.0, 0.0) - mul_is(e, kx));
-    }
-  }
-}
-
 // phi functions with the reference's branches (forward_step_cy_2d.pyx:123-148,
 // forward_step_cy_3d.pyx:54-59): Taylor series to degree 14 for |Z| < 0.5
 // (phi0 by Taylor in 2D, by exp in 3D), closed forms otherwise.
@@ -404,6 +359,20 @@ struct UpdArgs {
 
 enum { UPD_EULER = 0, UPD_ETD1 = 1, UPD_ETD2RK1 = 2, UPD_ETD2RK2 = 3 };
 
+// One ETD / Euler update of a mode (forward_step_cy_2d.pyx:21-120,
+// forward_step_cy_3d.pyx:17-124), Z = -coeff |k|^(2 order) dt with its phi values
+__device__ inline cpx<double> etd_combine(int kind, cpx<double> y, cpx<double> f1, cpx<double> f2,
+                                          double Z, double p0, double p1, double p2, double dt) {
+  if (Z == 0.0) {
+    if (kind == UPD_ETD2RK1) return y + scale(f2 - f1, dt / 2.0);
+    if (kind == UPD_ETD2RK2) return y + scale(f2, dt);
+    return y + scale(f1, dt);
+  }
+  if (kind == UPD_ETD1) return scale(y, p0) + scale(f1, p1 * dt);
+  if (kind == UPD_ETD2RK1) return y + scale(f2 - f1, p2 * dt);
+  return scale(y, p0) + scale(f2 - f1, 2.0 * p2 * dt) + scale(f1, p1 * dt);
+}
+
 // One thread per compact mode, all components: the phi functions depend on
 // the mode and the component's coefficient only, so components sharing a
 // coefficient (all velocity components; all field components) share them.
@@ -437,17 +406,113 @@ __global__ void update_kernel(const UpdArgs<T> a) {
       cprev = c;
       have = true;
     }
-    cpx<double> res;
-    if (Z == 0.0) {
-      if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, dt / 2.0);
-      else if (a.kind == UPD_ETD2RK2) res = y + scale(f2, dt);
-      else res = y + scale(f1, dt);
-    } else {
-      if (a.kind == UPD_ETD1) res = scale(y, p0) + scale(f1, p1 * dt);
-      else if (a.kind == UPD_ETD2RK1) res = y + scale(f2 - f1, p2 * dt);
-      else res = scale(y, p0) + scale(f2 - f1, 2.0 * p2 * dt) + scale(f1, p1 * dt);
+    a.out[comp][idx] = cT<T>(etd_combine(a.kind, y, f1, f2, Z, p0, p1, p2, dt));
+  }
+}
+
+// k-space epilogue of the fused right-hand side: 1/N, forcing, buoyancy,
+// projection, induction curl; optionally the ETD sweep that follows it.
+template <typename T>
+__global__ void epilogue_kernel(const EpiArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nk) return;
+  int col;
+  double kx, ky, kz;
+  kvec<double>(a.g, idx, col, kx, ky, kz);
+  const int nd = a.ndim;
+  const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
+  const bool upd = a.upd_kind >= 0;
+  const bool wd = !upd || a.write_deriv;
+  if (col >= a.g.Kx) {
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+      if (v >= nvar) break;
+      if (wd) a.deriv[v][idx] = mk(T(0), T(0));
+      if (upd) a.upd_out[v][idx] = mk(T(0), T(0));
     }
-    a.out[comp][idx] = cT<T>(res);
+    return;
+  }
+  const double s = 1.0 / a.npts;
+  const double kk[3] = {kx, ky, kz};
+  // arrays picked by ndim / direction without dynamic indexing (keeps the
+  // argument arrays and locals out of scratch memory)
+  const cpx<T>* stT = a.physics == PH_BOUSS ? (nd == 2 ? a.state[2] : a.state[3]) : nullptr;
+  const cpx<T>* stB = a.physics == PH_BOUSS
+                          ? (a.bdir == 0 ? a.state[0] : (a.bdir == 1 ? a.state[1] : a.state[2]))
+                          : nullptr;
+  cpx<double> F[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    F[i] = mk(0.0, 0.0);
+    if (i < nd) {
+      F[i] = scale(c64(a.U[i][idx]), s);
+      if (a.forcing[i]) F[i] = F[i] + c64(a.forcing[i][idx]);
+      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(c64(stT[idx]), a.galpha);
+    }
+  }
+  // projection F <- F - k (k.F) / |k|^2, |k|^2 = 0 replaced by 1
+  double k2 = kx * kx + ky * ky + kz * kz;
+  const double k2true = k2;
+  if (k2 == 0.0) k2 = 1.0;
+  cpx<double> kf = mk(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) kf = kf + scale(F[i], kk[i]);
+  cpx<double> D[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) D[i] = mk(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < nd) D[i] = F[i] - scale(kf, kk[i] / k2);
+  if (a.physics == PH_BOUSS) {
+    const cpx<T>* UT = nd == 2 ? a.U[2] : a.U[3];
+    const cpx<T>* fT = nd == 2 ? a.forcing[2] : a.forcing[3];
+    cpx<double> ft = scale(c64(UT[idx]), s) - scale(c64(stB[idx]), a.beta);
+    if (fT) ft = ft + c64(fT[idx]);
+    if (nd == 2) D[2] = ft;
+    else D[3] = ft;
+  } else if (a.physics == PH_MHD) {
+    if (nd == 3) {
+      const cpx<double> e0 = scale(c64(a.U[3][idx]), s), e1 = scale(c64(a.U[4][idx]), s),
+                        e2 = scale(c64(a.U[5][idx]), s);
+      // i k x E
+      D[3] = mul_is(e2, ky) - mul_is(e1, kz);
+      D[4] = mul_is(e0, kz) - mul_is(e2, kx);
+      D[5] = mul_is(e1, kx) - mul_is(e0, ky);
+    } else {
+      const cpx<double> e = scale(c64(a.U[2][idx]), s);
+      D[2] = mul_is(e, ky);
+      D[3] = mk(0.0, 0.0) - mul_is(e, kx);
+    }
+  }
+  if (wd) {
+#pragma unroll
+    for (int v = 0; v < 6; ++v)
+      if (v < nvar) a.deriv[v][idx] = cT<T>(D[v]);
+  }
+  if (!upd) return;
+  // the ETD sweep of this mode (update_kernel's arithmetic; every state
+  // entry of this mode was read above, so in-place outputs are safe)
+  double kp;
+  if (a.upd_order == 1) kp = k2true;
+  else if (a.upd_order == 2) kp = k2true * k2true;
+  else kp = pow(k2true, (double)a.upd_order);
+  const double dt = a.upd_dt;
+  double cprev = 0.0, Z = 0.0, p0 = 1.0, p1 = 1.0, p2 = 0.5;
+  bool have = false;
+#pragma unroll
+  for (int v = 0; v < 6; ++v) {
+    if (v >= nvar) break;
+    const double c = a.upd_coeff[v];
+    if (!have || c != cprev) {
+      Z = (c != 0.0 && a.upd_kind != UPD_EULER) ? (-(c * kp)) * dt : 0.0;
+      if (Z != 0.0) phi012(Z, nd == 2, p0, p1, p2);
+      cprev = c;
+      have = true;
+    }
+    const cpx<double> dv = c64(cT<T>(D[v]));  // the value an unfused sweep would read back
+    const cpx<double> y = c64(a.upd_y[v][idx]);
+    const cpx<double> f1 = (a.upd_kind >= UPD_ETD2RK1) ? c64(a.upd_f1[v][idx]) : dv;
+    a.upd_out[v][idx] = cT<T>(etd_combine(a.upd_kind, y, f1, dv, Z, p0, p1, p2, dt));
   }
 }
 

@@ -295,3 +295,39 @@ def test_steps_with_constant_forcing(shape, integ):
         got = download_fields(plan, 'hydro', nd, st)
         for c in range(nd):
             assert rel_l2(got['u'][c], ref['u'][c]) < 1e-12, (scale, c)
+
+
+FUSE_CASES = [('hydro', (32, 32), 'RK2mid'), ('hydro', (16, 16, 16), 'RK2trap'),
+              ('boussinesq', (32, 48), 'RK2mid'), ('boussinesq', (16, 8, 16), 'RK2trap'),
+              ('mhd', (16, 16, 16), 'RK2mid'), ('mhd', (32, 32), 'RK2trap')]
+
+
+@pytest.mark.parametrize("kind,shape,integ", FUSE_CASES, ids=str)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_fused_update_matches_separate_sweeps(kind, shape, integ, precision):
+    """ETD sweeps fused into the RHS epilogues give bit-identical states to
+    separate update launches (DDX_NO_FUSED_UPDATE=1)."""
+    import os
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("fused update", kind, shape, integ, precision)
+    prec = hp.F64 if precision == "f64" else hp.F32
+    p = {'nu': 2e-3, 'kappa': 1e-3, 'eta': 3e-3, 'viscosity_order': 1,
+         'boussinesq_direction': 'y' if nd == 2 else 'z'}
+    kf = kfields(kind, shape, 70, L)
+    integrator = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+    out = []
+    for fused in (True, False):
+        if not fused:
+            os.environ["DDX_NO_FUSED_UPDATE"] = "1"
+        try:
+            plan = hp.Plan(shape, L, prec)
+        finally:
+            os.environ.pop("DDX_NO_FUSED_UPDATE", None)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        plan.step(integrator, params_struct(hp, kind, nd, p), st, bufs[0], bufs[1], bufs[2], 4, 5e-3)
+        out.append([plan.download_k(b) for b in st])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
