@@ -132,7 +132,7 @@ class TrackMode(AnalysisTask):
                 for index in self.indexlist:
                     amps.append(kd[tuple(index)] if index is not None else 0.)
                 with open('%s%s_mode_amplitudes.dat' % (fname, field.ctrans[ci]), 'a') as f:
-                    f.write('%s\t' % data.time + '\t'.join(repr(a) for a in amps) + '\n')
+                    f.write('%s\t' % data.time + '\t'.join(repr(complex(a)) for a in amps) + '\n')
 
 
 def compute_spectrum(field, norm=1.):
