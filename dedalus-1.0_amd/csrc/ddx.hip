@@ -1375,6 +1375,21 @@ int ddx_kreduce(ddx_plan* p, const void* k0, const void* k1, int op, double* out
   return finish_reduce(p, nblk, 1, out);
 }
 
+int ddx_kreduce_members(ddx_plan* p, const void* k0, const void* k1, int op, double* out) {
+  if (check_plan(p) || !k0 || !out || (op == 1 && !k1)) return fail(DDX_ERR_ARG, "null argument");
+  if (op < 0 || op > 2) return fail(DDX_ERR_ARG, "bad reduction");
+  if (p->ndim != 2 || p->zfft) return fail(DDX_ERR_ARG, "per-member reductions need a 2D plan");
+  const int nb = p->az.n;
+  if (p->precision == DDX_F64)
+    hipLaunchKernelGGL((kreduce_members_kernel<double>), dim3(nb), dim3(256), 0, p->stream,
+                       (const cpx<double>*)k0, (const cpx<double>*)k1, kgeom(p), op, out);
+  else
+    hipLaunchKernelGGL((kreduce_members_kernel<float>), dim3(nb), dim3(256), 0, p->stream,
+                       (const cpx<float>*)k0, (const cpx<float>*)k1, kgeom(p), op, out);
+  LAUNCHCHK();
+  return 0;
+}
+
 int ddx_prof_enable(ddx_plan* p, int on) {
   if (check_plan(p)) return DDX_ERR_ARG;
   HIPCHK(hipStreamSynchronize(p->stream));

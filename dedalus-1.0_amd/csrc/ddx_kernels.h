@@ -707,28 +707,33 @@ __global__ void xreduce_kernel(const T* a, long n, int op, double* partial) {
 // Hermitian-weighted sums over compact k arrays (weight 1 on kx = 0, 2 elsewhere),
 // i.e. x-space means by Parseval.  op 0: sum |src|^2; 1: sum |i k x src|^2 summed
 // over the 2D curl of a (src0, src1) pair (enstrophy-like); 2: Re(src) at weight.
+// weighted contribution of compact mode i to a k-space reduction
+template <typename T>
+__device__ inline double kred_term(const cpx<T>* s0, const cpx<T>* s1, const KGeom& g, long i, int op) {
+  int col;
+  T kx, ky, kz;
+  kvec<T>(g, i, col, kx, ky, kz);
+  if (col >= g.Kx) return 0.0;
+  const double w = col == 0 ? 1.0 : 2.0;
+  if (op == 0) {
+    const cpx<T> v = s0[i];
+    return w * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  if (op == 1) {
+    // w_z = i kx s1 - i ky s0
+    const cpx<T> v = mul_is(s1[i], kx) - mul_is(s0[i], ky);
+    return w * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  return w * (double)s0[i].x;
+}
+
 template <typename T>
 __global__ void kreduce_kernel(const cpx<T>* s0, const cpx<T>* s1, KGeom g, long nk, int op,
                                double* partial) {
   __shared__ double sh[256];
   double acc = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x) {
-    int col;
-    T kx, ky, kz;
-    kvec<T>(g, i, col, kx, ky, kz);
-    if (col >= g.Kx) continue;
-    const double w = col == 0 ? 1.0 : 2.0;
-    if (op == 0) {
-      const cpx<T> v = s0[i];
-      acc += w * ((double)v.x * v.x + (double)v.y * v.y);
-    } else if (op == 1) {
-      // w_z = i kx s1 - i ky s0
-      const cpx<T> v = mul_is(s1[i], kx) - mul_is(s0[i], ky);
-      acc += w * ((double)v.x * v.x + (double)v.y * v.y);
-    } else {
-      acc += w * (double)s0[i].x;
-    }
-  }
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += (long)gridDim.x * blockDim.x)
+    acc += kred_term(s0, s1, g, i, op);
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -736,6 +741,29 @@ __global__ void kreduce_kernel(const cpx<T>* s0, const cpx<T>* s1, KGeom g, long
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// The same reduction per member of a batched plan: block b sums the modes
+// (ky, b, kx) of member b (the member is the z slab of the compact layout)
+// in a fixed order and writes out[b].
+template <typename T>
+__global__ void __launch_bounds__(256) kreduce_members_kernel(const cpx<T>* s0, const cpx<T>* s1,
+                                                              KGeom g, int op, double* out) {
+  __shared__ double sh[256];
+  const int b = blockIdx.x;
+  const long per = (long)g.Ky * g.Kxp;
+  double acc = 0.0;
+  for (long j = threadIdx.x; j < per; j += 256) {
+    const long ky = j / g.Kxp, c = j - ky * g.Kxp;
+    acc += kred_term(s0, s1, g, (ky * g.Kz + b) * g.Kxp + c, op);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = sh[0];
 }
 
 // real array conversion (x-space staging between the FP64 host boundary and FP32 plans)

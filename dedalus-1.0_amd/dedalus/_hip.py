@@ -91,6 +91,7 @@ SIGNATURES = {
     "ddx_xcombine": (_I, [_VP, _VP, _I, _PP, _PP, _DP, _I]),
     "ddx_xreduce": (_I, [_VP, _VP, _I, _DP]),
     "ddx_kreduce": (_I, [_VP, _VP, _VP, _I, _DP]),
+    "ddx_kreduce_members": (_I, [_VP, _VP, _VP, _I, _VP]),
     "ddx_prof_enable": (_I, [_VP, _I]),
     "ddx_prof_read": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.c_char_p, _I64P, _DP, _DP]),
     # include/ddx_closure.h
@@ -388,6 +389,16 @@ class Plan:
         _check(_lib.ddx_kreduce(self.handle, int(k0), int(k1) if k1 is not None else None,
                                 int(op), ctypes.byref(r)))
         return r.value
+
+    def kreduce_members(self, k0, op, k1=None, out=None):
+        """Per-member reductions of a 2D plan into a device buffer of
+        nbatch doubles (asynchronous); returns the buffer."""
+        nb = self.batch or 1
+        if out is None:
+            out = DevBuf(8 * nb)
+        _check(_lib.ddx_kreduce_members(self.handle, int(k0), int(k1) if k1 is not None else None,
+                                        int(op), ctypes.c_void_p(int(out))))
+        return out
 
     def rhs(self, params, state, deriv, forcing=None):
         fz = None

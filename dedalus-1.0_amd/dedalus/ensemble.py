@@ -146,13 +146,31 @@ class Ensemble:
     def sync(self):
         self.stream.synchronize()
 
-    def energies(self):
-        """<|u|^2>/2 per member (host)."""
+    def _member_reduce(self, op):
         out = []
         for g in self.groups:
-            for u, v in g.read():
-                out.append(0.5 * mean_square([u, v], g.shape))
-        return np.array(out)
+            if op == 0:
+                a = g.plan.kreduce_members(g.state[0], 0)
+                b = g.plan.kreduce_members(g.state[1], 0)
+                va, vb = np.empty(g.n), np.empty(g.n)
+                g.plan.d2h(va, a)
+                g.plan.d2h(vb, b)
+                out.append(va + vb)
+            else:
+                a = g.plan.kreduce_members(g.state[0], 1, g.state[1])
+                va = np.empty(g.n)
+                g.plan.d2h(va, a)
+                out.append(va)
+        return np.concatenate(out)
+
+    def energies(self):
+        """<|u|^2> / 2 per member, in member order (device reductions, one
+        download per group)."""
+        return 0.5 * self._member_reduce(0)
+
+    def enstrophies(self):
+        """<omega^2> / 2 per member (volume_average.py:191-200 for each member)."""
+        return 0.5 * self._member_reduce(1)
 
 
 def ensemble_768(stream=None):

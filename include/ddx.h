@@ -180,6 +180,12 @@ int ddx_xreduce(ddx_plan* plan, const void* x, int op, double* out);
  * |i kx k1 - i ky k0|^2 (2D vorticity of the pair), op 2 = sum Re(k0).
  * Synchronous; result in *out. */
 int ddx_kreduce(ddx_plan* plan, const void* k0, const void* k1, int op, double* out);
+/* ddx_kreduce per member of a 2D plan (batched or not): out is a device
+ * array of nbatch doubles (1 for a plain plan), written asynchronously on the
+ * plan's stream, member b summed in a fixed order.  No reference counterpart:
+ * the per-member diagnostics of an ensemble (volume_average.py:109-118,
+ * 191-200 for each member) without a host round trip per member. */
+int ddx_kreduce_members(ddx_plan* plan, const void* k0, const void* k1, int op, double* out);
 
 /* ---- fused right-hand side and step ---------------------------------------- */
 

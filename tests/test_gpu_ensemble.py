@@ -161,3 +161,30 @@ def test_ensemble_graph_matches_eager_and_oracle():
                 err = rel_l2(gsnap[j][c], ref['u'][c])
                 assert err < 2e-6, (shape, j, c, err)
             m += 1
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_member_reductions(precision):
+    """Per-member energy and enstrophy of an ensemble from device reductions
+    equal the host Parseval sums of each member's downloaded fields."""
+    hip()
+    from dedalus.ensemble import Ensemble
+    from dedalus.init_cond.seeded import mean_square
+    from dedalus.data_objects.kgrid import wavenumbers
+    case("member reductions", precision)
+    ens = Ensemble((((32, 32), 5), ((64, 48), 3)), nu=1e-3, dt=1e-3, precision=precision)
+    ens.seed_members(2000)
+    ens.advance(3)
+    ens.sync()
+    e = ens.energies()
+    z = ens.enstrophies()
+    assert e.shape == (8,) and z.shape == (8,)
+    m = 0
+    tol = 1e-13 if precision == "f64" else 1e-6
+    for g in ens.groups:
+        k, _, ksh = wavenumbers(g.shape, L2)
+        for u, v in g.read():
+            assert abs(e[m] - 0.5 * mean_square([u, v], g.shape)) <= tol * e[m]
+            w = 1j * k['x'] * v - 1j * k['y'] * u
+            assert abs(z[m] - 0.5 * mean_square([w], g.shape)) <= tol * z[m]
+            m += 1
