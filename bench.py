@@ -8,7 +8,7 @@ E(k) ~ k^4 exp(-2 (k/4)^2), rms 1 each, nu = eta = 1e-3, dt = 2.5e-4.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-Fields of the JSON line besides the driver's contract:
+Fields of the JSON line besides the standard ones:
   value         whole-job throughput of the timed region (K steps, state
                 resident in HBM, one HIP graph per step replayed K times)
   roofline      dominant kernel class (most GPU time): algorithmic bytes

@@ -1,5 +1,5 @@
 """Forced 2D turbulence -> (coarse state, sub-grid tendency) pairs -> trained
-closure -> coarse runs judged against the coarse-grained fine run.
+closure -> coarse runs compared with the coarse-grained fine run.
 
 Configuration (CLOSURE_512_64):
   fine    512^2 IncompressibleHydro, FP64, nu = 1e-4, RK2mid, dt = 2.5e-4,

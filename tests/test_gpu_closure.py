@@ -275,3 +275,27 @@ def test_network_trains():
         losses.append(loss.item())
     print("losses", losses[0], losses[-1])
     assert losses[-1] < 0.3 * losses[0]
+
+
+def test_inference_matches_network_forward():
+    """The preallocated inference pass (used inside captured coarse steps)
+    gives the training network's forward output bit for bit."""
+    d = dev()
+    from dedalus.closure import ClosureNet
+    from dedalus.closure.net import Inference, pack
+    case("inference vs forward")
+    net = ClosureNet(cin=3, cout=2, width=32, depth=5, seed=5).to(d)
+    g = torch.Generator().manual_seed(13)
+    with torch.no_grad():
+        for b in net.biases:
+            b.copy_((torch.randn(b.shape, generator=g) * 0.1).to(d))
+    B, H, W = 2, 64, 64
+    x = torch.randn(B, H, W, 3, generator=g).to(d)
+    with torch.no_grad():
+        ref = net(x)
+    inf = Inference(net, B, H, W)
+    pack([(x.data_ptr() + 4 * c, False) for c in range(3)], B * H * W, inf.x.shape[1],
+         pstride=3, bf16=inf.x, f32=False)
+    out = inf.run()
+    torch.cuda.synchronize()
+    assert torch.equal(out[:, :2].reshape(B, H, W, 2), ref)
