@@ -43,17 +43,19 @@ def _stream_of(stream):
 
 
 class CoarseGrainer:
-    """Fine plan N x N -> coarse plan M x M (float64), all launches on one stream."""
+    """Fine plan N x N -> coarse plan M x M (float64), all launches on one
+    stream; with ``batch`` both plans hold that many members."""
 
-    def __init__(self, fine_shape, factor, length=(2 * np.pi, 2 * np.pi), stream=None):
+    def __init__(self, fine_shape, factor, length=(2 * np.pi, 2 * np.pi), stream=None, batch=None):
         self.factor = int(factor)
         self.fine_shape = tuple(int(n) for n in fine_shape)
         if len(self.fine_shape) != 2 or any(n % self.factor for n in self.fine_shape):
             raise ValueError("2D grid with extents divisible by the factor required")
         self.coarse_shape = tuple(n // self.factor for n in self.fine_shape)
         self.length = tuple(length)
-        self.fine = _hip.Plan(self.fine_shape, self.length)
-        self.coarse = _hip.Plan(self.coarse_shape, self.length)
+        self.batch = int(batch) if batch else None
+        self.fine = _hip.Plan(self.fine_shape, self.length, batch=self.batch)
+        self.coarse = _hip.Plan(self.coarse_shape, self.length, batch=self.batch)
         self.stream = int(stream) if stream else None
         for p in (self.fine, self.coarse):
             p.set_stream(self.stream)
@@ -67,7 +69,10 @@ class CoarseGrainer:
         self._wk = C.alloc_k()
 
     def _box(self, src, dst):
+        # stacked members (B, ny, nx) are one (B ny, nx) image: ny % factor == 0,
+        # so no averaging block straddles two members
         ny, nx = self.fine_shape
+        ny *= self.batch or 1
         _hip._check(_hip._lib.ddx_box_filter(_stream_of(self.stream), ny, nx, self.factor, 1,
                                              ctypes.c_void_p(int(src)), ctypes.c_void_p(int(dst))))
 

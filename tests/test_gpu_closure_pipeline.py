@@ -172,3 +172,40 @@ def test_batched_closure_run_matches_single_runs():
         r1.advance(5)
         for c in range(2):
             assert np.array_equal(got[c][m], C.download_k(r1.state[c])), (m, c)
+
+
+def test_batched_pairs_match_single():
+    """Sub-grid pairs of a batched coarse-grainer equal per-member single
+    coarse-grainers bit for bit."""
+    import torch
+    from dedalus.closure.sgs import CoarseGrainer, initial_velocity
+    from dedalus import _hip
+    _hip.init()
+    case("batched sgs pairs")
+    fine, f, nb = (128, 128), 4, 3
+    s = torch.cuda.Stream()
+    cb = CoarseGrainer(fine, f, L2, stream=s.cuda_stream, batch=nb)
+    u = [initial_velocity(fine, L2, 40 + m) for m in range(nb)]
+    st = []
+    for c in range(2):
+        b = cb.fine.alloc_k()
+        cb.fine.upload_k(b, np.stack([u[m][c] for m in range(nb)]))
+        st.append(b)
+    ubar = [cb.coarse.alloc_k() for _ in range(2)]
+    tau = [cb.coarse.alloc_k() for _ in range(2)]
+    cb.pair(st, ubar, tau)
+    gu = [cb.coarse.download_k(b) for b in ubar]
+    gt = [cb.coarse.download_k(b) for b in tau]
+    c1 = CoarseGrainer(fine, f, L2, stream=s.cuda_stream)
+    for m in range(nb):
+        st1 = []
+        for c in range(2):
+            b = c1.fine.alloc_k()
+            c1.fine.upload_k(b, u[m][c])
+            st1.append(b)
+        ub1 = [c1.coarse.alloc_k() for _ in range(2)]
+        ta1 = [c1.coarse.alloc_k() for _ in range(2)]
+        c1.pair(st1, ub1, ta1)
+        for c in range(2):
+            assert np.array_equal(gu[c][m], c1.coarse.download_k(ub1[c])), (m, c)
+            assert np.array_equal(gt[c][m], c1.coarse.download_k(ta1[c])), (m, c)
