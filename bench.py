@@ -188,7 +188,7 @@ def ensemble_cpu_baseline():
     }
 
 
-def closure_summary(batches=(1, 64, 1024), iters=3000, t_eval=10.0):
+def closure_summary(batches=(1, 64, 1024), iters=None, t_eval=10.0):
     """The learned closure on 512^2 -> 64^2 forced 2D turbulence: pairs from
     the fine solver, training, spectrum error of coarse runs against the
     coarse-grained fine run, throughput against the plain 512^2 run, and the
@@ -203,11 +203,12 @@ def closure_summary(batches=(1, 64, 1024), iters=3000, t_eval=10.0):
     t_pairs = time.perf_counter() - t0
     log("closure: %d pairs in %.1f s" % (X.shape[0], t_pairs))
     net = ClosureNet(cin=3, cout=2, width=cfg["width"], depth=cfg["depth"], seed=0).cuda()
+    iters = iters or cfg["train_iters"]
     t1 = time.perf_counter()
     scales, hist = P.train(net, X, Y, iters=iters)
     t_train = time.perf_counter() - t1
     log("closure: trained %d iterations in %.1f s" % (iters, t_train))
-    res = P.evaluate(pipe, net, scales, T=t_eval)
+    res = P.evaluate(pipe, net, scales, T=t_eval, seeds=cfg["eval_seeds"])
     log("closure: spectrum error %s" % res["err"])
     t_fine = P.time_fine_steps(pipe)
     ratio = cfg["dt_ratio"]
@@ -243,7 +244,8 @@ def closure_summary(batches=(1, 64, 1024), iters=3000, t_eval=10.0):
         "final_loss": round(hist[-1][1], 4), "val_loss": round(hist[-1][2], 4),
         "spectrum_err": {k: round(v, 4) for k, v in res["err"].items()},
         "spectrum_err_def": "mean over 1 <= k <= %d of |E_model/E_truth - 1|, shell sums "
-                            "time-averaged over t in [%g, %g]" % (res["kc"], t_eval / 2, t_eval),
+                            "averaged over %d held-out seeds and t in [%g, %g]"
+                            % (res["kc"], len(res["seeds"]), t_eval / 2, t_eval),
         "fine_ms_per_step": round(t_fine, 4),
         "throughput": thr,
         "conv_roofline": {"bound": "mfma", "kernel": "conv3x3 forward (5 layers, 256 images 64^2)",

@@ -7,8 +7,9 @@ Configuration (CLOSURE_512_64):
           and phases fixed by a seed (the VelocityForcing hook,
           dedalus/physics/physics.py:524-525, 569-572)
   coarse  64^2 (8 x 8 box filter), same nu, dt = 2e-3, forcing = B(f)
-  pairs   every 80 fine steps (0.02 time units) after a spin-up of 5 time
-          units, 500 per seed
+  pairs   8 seeds (one batched fine run), every 80 fine steps (0.02 time
+          units) after a spin-up of 5 time units, 500 per seed
+  training 8000 Adam iterations, batch 32
   network (u, v, omega) -> tau (2 components), 3x3 periodic convolutions,
           widths 32, 5 layers, GELU, BF16 MFMA with FP32 accumulation
 
@@ -36,8 +37,9 @@ from .sgs import CoarseGrainer, band_forcing, features_x, initial_velocity, shel
 CLOSURE_512_64 = dict(
     fine_shape=(512, 512), factor=8, length=(2 * np.pi, 2 * np.pi), nu=1e-4,
     dt_fine=2.5e-4, dt_ratio=8, forcing=dict(kmin=3.5, kmax=4.5, rms=0.1), forcing_seed=7,
-    spinup=5.0, sample_every=80, pairs_per_seed=500, train_seeds=(101, 102, 103, 104),
-    eval_seed=201, width=32, depth=5,
+    spinup=5.0, sample_every=80, pairs_per_seed=500,
+    train_seeds=(101, 102, 103, 104, 105, 106, 107, 108), train_iters=8000,
+    eval_seed=201, eval_seeds=(201, 202, 203, 204), width=32, depth=5,
 )
 
 
@@ -66,6 +68,7 @@ class Pipeline:
         L = cfg["length"]
         fc = cfg["forcing"]
         fk = band_forcing(cfg["fine_shape"], L, cfg["forcing_seed"], fc["kmin"], fc["kmax"], fc["rms"])
+        self._fk_host = fk
         with torch.cuda.stream(self.stream):
             self.f_fine = [F.alloc_k() for _ in range(2)]
             for b, a in zip(self.f_fine, fk):
@@ -100,10 +103,17 @@ class Pipeline:
         return float(np.sqrt(m))
 
     # ---- pairs -----------------------------------------------------------
-    def generate_pairs(self, seeds=None, log=None):
-        """(X [N, m, m, 3], Y [N, m, m, 2]) float32 on the GPU, N = seeds x pairs_per_seed."""
+    def generate_pairs(self, seeds=None, log=None, batched=True):
+        """(X [N, m, m, 3], Y [N, m, m, 2]) float32 on the GPU, N = seeds x pairs_per_seed.
+
+        batched: one fine run per seed, all seeds as members of one batched
+        plan; pair i = j * len(seeds) + s is sample j of seed s (time-major, so
+        the last pairs are the latest samples of every seed).  Otherwise the
+        seeds run one after another and pair i = s * pairs_per_seed + j."""
         cfg = self.cfg
         seeds = cfg["train_seeds"] if seeds is None else seeds
+        if batched and len(seeds) > 1:
+            return self._generate_pairs_batched(list(seeds), log)
         C = self.cg.coarse
         m = self.coarse_shape
         npix = m[0] * m[1]
@@ -143,6 +153,57 @@ class Pipeline:
             torch.cuda.current_stream().synchronize()
         return X, Y, info
 
+    def _generate_pairs_batched(self, seeds, log=None):
+        cfg = self.cfg
+        nb = len(seeds)
+        cg = CoarseGrainer(cfg["fine_shape"], cfg["factor"], cfg["length"],
+                           stream=self.stream.cuda_stream, batch=nb)
+        F, C = cg.fine, cg.coarse
+        m = self.coarse_shape
+        npix = nb * m[0] * m[1]
+        P = cfg["pairs_per_seed"]
+        dev = torch.device("cuda", torch.cuda.current_device())
+        X = torch.empty((P * nb, m[0], m[1], 3), dtype=torch.float32, device=dev)
+        Y = torch.empty((P * nb, m[0], m[1], 2), dtype=torch.float32, device=dev)
+        nspin = int(round(cfg["spinup"] / cfg["dt_fine"]))
+        with torch.cuda.stream(self.stream):
+            f = [F.alloc_k() for _ in range(2)]
+            st = [F.alloc_k() for _ in range(2)]
+            u0 = [initial_velocity(cfg["fine_shape"], cfg["length"], sd) for sd in seeds]
+            for c in range(2):
+                F.upload_k(f[c], np.stack([self._fk_host[c]] * nb))
+                F.upload_k(st[c], np.stack([u[c] for u in u0]))
+            bufs = [[F.alloc_k() for _ in range(2)] for _ in range(3)]
+            ubar = [C.alloc_k() for _ in range(2)]
+            tau = [C.alloc_k() for _ in range(2)]
+            xb = [C.alloc_x() for _ in range(3)]
+
+            def advance(n):
+                F.step(_hip.RK2MID, self.prm, st, bufs[0], bufs[1], bufs[2], int(n), cfg["dt_fine"],
+                       forcing=f)
+
+            advance(nspin)
+            umax = []
+            for j in range(P):
+                if j:
+                    advance(cfg["sample_every"])
+                cg.pair(st, ubar, tau)
+                cg.features_x(ubar, xb)
+                pack([(int(b), True) for b in xb], npix, 3, bf16=False,
+                     f32=X[j * nb:(j + 1) * nb].reshape(npix, 3))
+                C.bwd(tau[0], xb[0])
+                C.bwd(tau[1], xb[1])
+                pack([(int(xb[0]), True), (int(xb[1]), True)], npix, 2, bf16=False,
+                     f32=Y[j * nb:(j + 1) * nb].reshape(npix, 2))
+                if j % max(1, P // 4) == 0:
+                    umax.append(self.max_speed(F, st))
+            torch.cuda.current_stream().synchronize()
+        cfl = max(umax) * cfg["dt_fine"] / (cfg["length"][0] / cfg["fine_shape"][1])
+        info = [dict(seeds=list(seeds), max_speed=max(umax), cfl=cfl)]
+        if log:
+            log("pairs: %d seeds batched, max |u| %.3f, CFL %.3f" % (nb, max(umax), cfl))
+        return X, Y, info
+
     # ---- coarse runs -----------------------------------------------------
     def coarse_run(self, net=None, scales=None, batch=None):
         return CoarseRun(self, net, scales, batch)
@@ -150,7 +211,7 @@ class Pipeline:
 
 def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=None):
     """Adam on the normalised MSE; returns (scales, history).  The last
-    val_frac of the pairs (the end of the last seed's run) is held out."""
+    val_frac of the pairs (the latest samples) is held out."""
     n = X.shape[0]
     nval = max(1, int(round(n * val_frac)))
     ntr = n - nval
@@ -282,19 +343,21 @@ def spectrum_of(plan, bufs, shape, length):
     return shell_spectrum([plan.download_k(b) for b in bufs], shape, length)[1]
 
 
-def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None, seed=None,
+def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None, seeds=None,
              log=None):
     """Run the fine truth and coarse models (with and without closure) from the
-    same coarse-grained state; return spectra and errors.
+    same coarse-grained states; return spectra and errors.
 
-    Spectra are Hermitian-weighted shell sums (k = 0..kmax of the coarse
-    grid), time-averaged over snapshots t >= avg_from (default T/2).  The
+    seeds: evaluation seeds (default: the configuration's eval_seed), run
+    together as members of batched fine and coarse plans.  Spectra are
+    Hermitian-weighted shell sums (k = 0..kmax of the coarse grid), averaged
+    over the members and over snapshots t >= avg_from (default T/2).  The
     error is the mean over 1 <= k <= kc (kc = the coarse 2/3 cutoff) of
     |E_model(k) / E_truth(k) - 1|."""
     cfg = pipe.cfg
-    seed = cfg["eval_seed"] if seed is None else seed
+    seeds = list(seeds) if seeds else [cfg["eval_seed"]]
+    nb = len(seeds)
     avg_from = T / 2 if avg_from is None else avg_from
-    F, C = pipe.cg.fine, pipe.cg.coarse
     L = cfg["length"]
     cs = pipe.coarse_shape
     ratio = cfg["dt_ratio"]
@@ -302,32 +365,49 @@ def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None,
     nsnap = int(round(T / snap_every))
     csteps = int(round(snap_every / dtc))
     nspin = int(round(cfg["spinup"] / cfg["dt_fine"]))
-    ubar = [C.alloc_k() for _ in range(2)]
-    runs = {"none": pipe.coarse_run(None)}
+    cg = CoarseGrainer(cfg["fine_shape"], cfg["factor"], L, stream=pipe.stream.cuda_stream, batch=nb)
+    F, C = cg.fine, cg.coarse
+    runs = {"none": pipe.coarse_run(None, batch=nb)}
     if net is not None:
-        runs["closure"] = pipe.coarse_run(net, scales)
+        runs["closure"] = pipe.coarse_run(net, scales, batch=nb)
+
+    def member_spectra(plan, bufs):
+        arrs = [plan.download_k(b) for b in bufs]
+        return np.mean([shell_spectrum([a[m] for a in arrs], cs, L)[1] for m in range(nb)], axis=0)
+
     with torch.cuda.stream(pipe.stream):
-        st, bufs = pipe.fine_state(seed)
-        pipe.fine_advance(st, bufs, nspin)
+        f = [F.alloc_k() for _ in range(2)]
+        st = [F.alloc_k() for _ in range(2)]
+        u0 = [initial_velocity(cfg["fine_shape"], L, sd) for sd in seeds]
         for c in range(2):
-            pipe.cg.filter_k(st[c], ubar[c])
+            F.upload_k(f[c], np.stack([pipe._fk_host[c]] * nb))
+            F.upload_k(st[c], np.stack([u[c] for u in u0]))
+        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(3)]
+
+        def advance(n):
+            F.step(_hip.RK2MID, pipe.prm, st, bufs[0], bufs[1], bufs[2], int(n), cfg["dt_fine"],
+                   forcing=f)
+
+        ubar = [C.alloc_k() for _ in range(2)]
+        advance(nspin)
+        for c in range(2):
+            cg.filter_k(st[c], ubar[c])
         for r in runs.values():
             r.set_state(ubar)
             r.capture()
             r.set_state(ubar)
         spec = {k: [] for k in ["truth"] + list(runs)}
         times = []
-        for s in range(nsnap):
-            pipe.fine_advance(st, bufs, csteps * ratio)
+        for sn in range(nsnap):
+            advance(csteps * ratio)
             for c in range(2):
-                pipe.cg.filter_k(st[c], ubar[c])
+                cg.filter_k(st[c], ubar[c])
             for r in runs.values():
                 r.advance(csteps)
-            t = (s + 1) * csteps * dtc
-            times.append(t)
-            spec["truth"].append(spectrum_of(C, ubar, cs, L))
+            times.append((sn + 1) * csteps * dtc)
+            spec["truth"].append(member_spectra(C, ubar))
             for name, r in runs.items():
-                spec[name].append(spectrum_of(C, r.state, cs, L))
+                spec[name].append(member_spectra(r.C, r.state))
         C.sync()
     times = np.array(times)
     sel = times >= avg_from - 1e-12
@@ -337,7 +417,8 @@ def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None,
     err = {k: float(np.mean(np.abs(mean[k][ks] / mean["truth"][ks] - 1.0)))
            for k in runs}
     ens = {k: [float(np.sum(np.arange(len(e)) ** 2 * e)) for e in v] for k, v in spec.items()}
-    res = dict(times=times.tolist(), err=err, kc=int(kc), spectra={k: v.tolist() for k, v in mean.items()},
+    res = dict(times=times.tolist(), err=err, kc=int(kc), seeds=seeds,
+               spectra={k: v.tolist() for k, v in mean.items()},
                enstrophy=ens, energy={k: [float(e.sum()) for e in v] for k, v in spec.items()})
     if log:
         log("evaluate: spectrum error " + ", ".join("%s %.3f" % kv for kv in err.items()))

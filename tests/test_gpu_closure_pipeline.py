@@ -136,10 +136,16 @@ def test_closure_reduces_spectrum_error_small():
     net = ClosureNet(cin=3, cout=2, width=16, depth=3, seed=0).cuda()
     scales, hist = P.train(net, X, Y, iters=1500, batch=8)
     assert hist[-1][1] < 0.5 * hist[0][1]
-    res = P.evaluate(pipe, net, scales, T=4.0)
+    res = P.evaluate(pipe, net, scales, T=4.0, seeds=(21, 22))
     print("spectrum error", res["err"])
     assert np.isfinite(res["err"]["closure"])
     assert res["err"]["closure"] < 0.5 * res["err"]["none"]
+    # the batched evaluation is the mean of the members' own evaluations
+    single = [P.evaluate(pipe, net, scales, T=4.0, seeds=(s,)) for s in (21, 22)]
+    for name in ("truth", "none", "closure"):
+        both = np.array(res["spectra"][name])
+        avg = 0.5 * (np.array(single[0]["spectra"][name]) + np.array(single[1]["spectra"][name]))
+        assert np.allclose(both, avg, rtol=1e-10, atol=1e-14 * avg.max()), name
 
 
 def test_batched_closure_run_matches_single_runs():
