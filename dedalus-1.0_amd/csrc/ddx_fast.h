@@ -46,15 +46,19 @@ constexpr int exch_size() {
 }
 
 // Butterflies of one pass with twiddles from a per-thread register cache
-// (c[q*R + r] = forward twiddle of butterfly q, leg r; conjugated for D > 0).
-template <typename T, int D, int E, int R, int P>
+// (conjugated for D > 0).  POW = false: c[q*R + r] = forward twiddle of
+// butterfly q, leg r.  POW = true: c[q] = the leg-1 twiddle w of butterfly q
+// and leg r uses w^r, formed by r - 1 complex products (about r ulp from the
+// table value); the cache then takes E/R registers pairs instead of E.
+template <typename T, int D, int E, int R, int P, bool POW>
 __device__ __forceinline__ void reg_pass_cached(cpx<T> (&v)[E], const cpx<T> (&c)[E]) {
 #pragma unroll
   for (int q = 0; q < E / R; ++q) {
     if (P > 1) {
+      cpx<T> w = POW ? c[q] : c[q * R + 1];
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const cpx<T> w = c[q * R + r];
+        if (r > 1) w = POW ? w * c[q] : c[q * R + r];
         v[q * R + r] = v[q * R + r] * mk(w.x, D < 0 ? w.y : -w.y);
       }
     }
@@ -121,8 +125,9 @@ __device__ __forceinline__ void run_passes(cpx<T> (&v)[PL::E], int lane, int c, 
 }
 
 // Register twiddle cache of a plan: pass I of the (REV-ordered) radix
-// sequence gets E entries, c[I][q*R + r] = tw[r * (b % P) * N/(P R)].
-template <typename T, int N, class PL, bool REV, int I = 0, int P = 1>
+// sequence gets E entries, c[I][q*R + r] = tw[r * (b % P) * N/(P R)]
+// (POW = false), or E/R entries c[I][q] = tw[(b % P) * N/(P R)] (POW = true).
+template <typename T, int N, class PL, bool REV, bool POW, int I = 0, int P = 1>
 __device__ __forceinline__ void fill_twiddles(cpx<T> (&c)[PL::NP][PL::E], int lane,
                                               const cpx<T>* __restrict__ tw) {
   constexpr int NP = PL::NP;
@@ -130,19 +135,24 @@ __device__ __forceinline__ void fill_twiddles(cpx<T> (&c)[PL::NP][PL::E], int la
 #pragma unroll
   for (int q = 0; q < PL::E / R; ++q) {
     const int k = (lane + PL::LANES * q) % P;
+    if (POW) {
+      c[I][q] = P > 1 ? tw[k * (N / (P * R))] : mk(T(1), T(0));
+    } else {
 #pragma unroll
-    for (int r = 0; r < R; ++r) c[I][q * R + r] = (P > 1 && r > 0) ? tw[r * k * (N / (P * R))] : mk(T(1), T(0));
+      for (int r = 0; r < R; ++r)
+        c[I][q * R + r] = (P > 1 && r > 0) ? tw[r * k * (N / (P * R))] : mk(T(1), T(0));
+    }
   }
-  if constexpr (I + 1 < NP) fill_twiddles<T, N, PL, REV, I + 1, P * R>(c, lane, tw);
+  if constexpr (I + 1 < NP) fill_twiddles<T, N, PL, REV, POW, I + 1, P * R>(c, lane, tw);
 }
 
 // run_passes with twiddles from a register cache filled by fill_twiddles
-template <typename T, int D, int N, class PL, bool REV, int WS, int I = 0, int P = 1>
+template <typename T, int D, int N, class PL, bool REV, int WS, bool POW, int I = 0, int P = 1>
 __device__ __forceinline__ void run_passes_cached(cpx<T> (&v)[PL::E], int lane, int c, cpx<T>* buf,
                                                   const cpx<T> (&twc)[PL::NP][PL::E]) {
   constexpr int NP = PL::NP;
   constexpr int R = PL::R[REV ? NP - 1 - I : I];
-  reg_pass_cached<T, D, PL::E, R, P>(v, twc[I]);
+  reg_pass_cached<T, D, PL::E, R, P, POW>(v, twc[I]);
   if constexpr (I + 1 < NP) {
     constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
 #pragma unroll
@@ -157,7 +167,7 @@ __device__ __forceinline__ void run_passes_cached(cpx<T> (&v)[PL::E], int lane, 
       for (int r = 0; r < R2; ++r)
         v[q * R2 + r] = buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c];
     __syncthreads();
-    run_passes_cached<T, D, N, PL, REV, WS, I + 1, P * R>(v, lane, c, buf, twc);
+    run_passes_cached<T, D, N, PL, REV, WS, POW, I + 1, P * R>(v, lane, c, buf, twc);
   }
 }
 
@@ -204,34 +214,49 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
   const int cx = a.kx - 1;
   const cpx<T>* __restrict__ tw = a.tw;
   constexpr bool UNI = uniform_radix<PL>();
+  // FP64: leg-1 twiddles only (powers formed in the butterflies), which frees
+  // the registers the 12 x-space fields of the MHD row need
+  constexpr bool POW = sizeof(T) == 8;
   cpx<T> twi[PL::NP][E];
-  fill_twiddles<T, N, PL, false>(twi, lane, tw);
+  fill_twiddles<T, N, PL, false, POW>(twi, lane, tw);
 
+  // Loads are unconditional, from clamped in-range addresses (the last row
+  // for a team past the end, entry 0 for positions the 2/3 rule leaves
+  // empty), and the unused values are zeroed by selects afterwards: loads
+  // under lane-dependent branches each end in a wait for memory, which
+  // serialises a row's dozens of loads.
+  const long rl = live ? r : nrows - 1;
   T q[NIN][E];
 #pragma unroll
   for (int p = 0; p < NPI; ++p) {
     cpx<T> v[E];
     const bool hasb = (2 * p + 1 < NIN);
-    const cpx<T>* __restrict__ ia = a.in[2 * p] + r * a.kxp;
-    const cpx<T>* __restrict__ ib = a.in[hasb ? 2 * p + 1 : 2 * p] + r * a.kxp;
+    const cpx<T>* __restrict__ ia = a.in[2 * p] + rl * a.kxp;
+    const cpx<T>* __restrict__ ib = a.in[hasb ? 2 * p + 1 : 2 * p] + rl * a.kxp;
+    cpx<T> la[E], lb[E];
 #pragma unroll
     for (int qq = 0; qq < E / R0; ++qq)
 #pragma unroll
       for (int rr = 0; rr < R0; ++rr) {
         const int m = in_pos<N, LANES, R0>(lane, qq, rr);
-        cpx<T> va = mk(T(0), T(0)), vb = mk(T(0), T(0));
-        if (!live) {
-        } else if (m <= cx) {
-          va = ia[m];
-          if (hasb) vb = ib[m];
-          if (m == 0) { va.y = T(0); vb.y = T(0); }
-        } else if (m >= N - cx) {
-          va = conj(ia[N - m]);
-          if (hasb) vb = conj(ib[N - m]);
-        }
-        v[qq * R0 + rr] = mk(va.x - vb.y, va.y + vb.x);
+        const int idx = m <= cx ? m : (m >= N - cx ? N - m : 0);
+        la[qq * R0 + rr] = ia[idx];
+        lb[qq * R0 + rr] = hasb ? ib[idx] : mk(T(0), T(0));
       }
-    run_passes_cached<T, 1, N, PL, false, 1>(v, lane, 0, buf, twi);
+#pragma unroll
+    for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+      for (int rr = 0; rr < R0; ++rr) {
+        const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+        const bool keep = live && (m <= cx || m >= N - cx);
+        const bool mirror = m >= N - cx;
+        const cpx<T> va = la[qq * R0 + rr], vb = lb[qq * R0 + rr];
+        // conj for mirrored positions; the k = 0 entry is real
+        const T ay = m == 0 ? T(0) : (mirror ? -va.y : va.y);
+        const T by = m == 0 ? T(0) : (mirror ? -vb.y : vb.y);
+        v[qq * R0 + rr] = keep ? mk(va.x - by, ay + vb.x) : mk(T(0), T(0));
+      }
+    run_passes_cached<T, 1, N, PL, false, 1, POW>(v, lane, 0, buf, twi);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       q[2 * p][e] = v[e].x;
@@ -260,7 +285,7 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
 #pragma unroll
       for (int e = 0; e < E; ++e) twf[i][e] = twi[i][e];
   } else {
-    fill_twiddles<T, N, PL, true>(twf, lane, tw);
+    fill_twiddles<T, N, PL, true, POW>(twf, lane, tw);
   }
 #pragma unroll
   for (int p = 0; p < NPO; ++p) {
@@ -268,7 +293,7 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
     const bool hasb = (2 * p + 1 < NOUT);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = mk(o[2 * p][e], hasb ? o[2 * p + 1][e] : T(0));
-    run_passes_cached<T, -1, N, PL, true, 1>(v, lane, 0, buf, twf);
+    run_passes_cached<T, -1, N, PL, true, 1, POW>(v, lane, 0, buf, twf);
     constexpr int RL = PL::R[0];
 #pragma unroll
     for (int qq = 0; qq < E / RL; ++qq)
@@ -299,6 +324,78 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
 // column kernel: W columns per workgroup, LANES threads per column
 // ---------------------------------------------------------------------------
 
+// First-pass inputs of one column: the stored rows (compact layouts leave out
+// the rows the 2/3 rule removes), or with SPECTRAL the recipe's NT terms
+// coef * (i k)^[dim >= 0] * src evaluated on the fly.  Every load is issued
+// unconditionally, from a clamped in-range address (column 0 past the last
+// column, row 0 for rows left out), and unused values are zeroed by a select
+// afterwards: loads under lane-dependent or per-term branches each end in a
+// wait for memory, which serialises the column's loads.
+template <typename T, int N, class PL, bool SPECTRAL, int NT>
+__device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>& a, int lane, int f,
+                                           long slab, int col, bool live) {
+  constexpr int LANES = PL::LANES, E = PL::E, R0 = PL::R[0];
+  const int colc = live ? col : 0;
+  // per term: source, coefficient and wavenumber table (the kx table, valid
+  // in every pass, stands in for terms without a derivative)
+  constexpr int NTA = NT > 0 ? NT : 1;
+  const cpx<T>* srcp[NTA];
+  const double* kp[NTA];
+  bool deriv[NTA], byrow[NTA], byslab[NTA];
+  T coef[NTA];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const Term tm = a.rec[f].t[t];
+    srcp[t] = a.src[tm.src];
+    deriv[t] = tm.dim >= 0;
+    byrow[t] = tm.dim > 0 && tm.dim == a.row_dim;
+    byslab[t] = tm.dim > 0 && tm.dim != a.row_dim;
+    kp[t] = byrow[t] ? a.k_row : (byslab[t] ? a.k_slab : a.k_col);
+    coef[t] = T(tm.coef);
+  }
+  cpx<T> ld[NTA][E];
+  double kv[NTA][E];
+  bool ok[E];
+#pragma unroll
+  for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < R0; ++rr) {
+      const int e = qq * R0 + rr;
+      const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+      int row;
+      if (a.in_c < 0) row = m;
+      else if (m <= a.in_c) row = m;
+      else if (m >= N - a.in_c) row = m - (N - a.in_rows);
+      else row = -1;
+      ok[e] = live && row >= 0;
+      const int rowc = row >= 0 ? row : 0;
+      const long off = slab * a.in_ss + (long)rowc * a.in_rs + colc;
+      if (SPECTRAL) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          ld[t][e] = srcp[t][off];
+          kv[t][e] = kp[t][byrow[t] ? (long)rowc : (byslab[t] ? slab : (long)colc)];
+        }
+      } else {
+        ld[0][e] = a.in[f][off];
+      }
+    }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    cpx<T> x = mk(T(0), T(0));
+    if (SPECTRAL) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const cpx<T> sv = deriv[t] ? mul_is(ld[t][e], T(kv[t][e])) : ld[t][e];
+        x = x + scale(sv, coef[t]);
+      }
+    } else {
+      x = ld[0][e];
+    }
+    v[e] = ok[e] ? x : mk(T(0), T(0));
+  }
+}
+
 template <typename T, int D, int N, bool SPECTRAL>
 __global__ void __launch_bounds__(ColPlan<N>::LANES * ColPlan<N>::W)
 col_fft_fast(const ColArgs<T> a) {
@@ -314,40 +411,14 @@ col_fft_fast(const ColArgs<T> a) {
   const int col = tile * W + c;
   const bool live = col < a.ncols;
   cpx<T> v[E];
-#pragma unroll
-  for (int qq = 0; qq < E / R0; ++qq)
-#pragma unroll
-    for (int rr = 0; rr < R0; ++rr) {
-      const int m = in_pos<N, LANES, R0>(lane, qq, rr);
-      cpx<T> x = mk(T(0), T(0));
-      int row;
-      if (a.in_c < 0) row = m;
-      else if (m <= a.in_c) row = m;
-      else if (m >= N - a.in_c) row = m - (N - a.in_rows);
-      else row = -1;
-      if (live && row >= 0) {
-        const long off = slab * a.in_ss + (long)row * a.in_rs + col;
-        if (SPECTRAL) {
-          const int nterm = a.rec[f].nterm;
-#pragma unroll
-          for (int t = 0; t < 2; ++t) {
-            if (t < nterm) {
-              const Term tm = a.rec[f].t[t];
-              cpx<T> s = a.src[tm.src][off];
-              if (tm.dim >= 0) {
-                const double kk = (tm.dim == 0) ? a.k_col[col]
-                                  : (tm.dim == a.row_dim ? a.k_row[row] : a.k_slab[slab]);
-                s = mul_is(s, T(kk));
-              }
-              x = x + scale(s, T(tm.coef));
-            }
-          }
-        } else {
-          x = a.in[f][off];
-        }
-      }
-      v[qq * R0 + rr] = x;
-    }
+  if (SPECTRAL) {
+    // the term count is uniform: one straight-line gather per count
+    const int nterm = a.rec[f].nterm;
+    if (nterm >= 2) col_gather<T, N, PL, true, 2>(v, a, lane, f, slab, col, live);
+    else col_gather<T, N, PL, true, 1>(v, a, lane, f, slab, col, live);
+  } else {
+    col_gather<T, N, PL, false, 0>(v, a, lane, f, slab, col, live);
+  }
   run_passes<T, D, N, PL, false, W>(v, lane, c, lds, a.tw);
   if (!live) return;
 #pragma unroll
