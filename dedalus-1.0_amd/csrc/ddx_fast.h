@@ -197,6 +197,26 @@ template <> struct PhysIO<PH_BOUSS, 2> { static constexpr int NIN = 6, NOUT = 3;
 template <> struct PhysIO<PH_MHD, 3>   { static constexpr int NIN = 12, NOUT = 6; };
 template <> struct PhysIO<PH_MHD, 2>   { static constexpr int NIN = 6, NOUT = 3; };
 
+// Loads of input pair p of a row (fields 2p and 2p + 1) at the first-pass
+// positions of this lane, from clamped in-range entries
+template <typename T, int N, class PL, int NIN>
+__device__ __forceinline__ void row_loads(cpx<T> (&la)[PL::E], cpx<T> (&lb)[PL::E], const RowArgs<T>& a,
+                                          int p, long rl, int lane, int cx) {
+  constexpr int LANES = PL::LANES, E = PL::E, R0 = PL::R[0];
+  const bool hasb = (2 * p + 1 < NIN);
+  const cpx<T>* __restrict__ ia = a.in[2 * p] + rl * a.kxp;
+  const cpx<T>* __restrict__ ib = a.in[hasb ? 2 * p + 1 : 2 * p] + rl * a.kxp;
+#pragma unroll
+  for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < R0; ++rr) {
+      const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+      const int idx = m <= cx ? m : (m >= N - cx ? N - m : 0);
+      la[qq * R0 + rr] = ia[idx];
+      lb[qq * R0 + rr] = hasb ? ib[idx] : mk(T(0), T(0));
+    }
+}
+
 template <typename T, int N, int PH, int ND>
 __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nrows) {
   using PL = RowPlan<N>;
@@ -222,27 +242,19 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
 
   // Loads are unconditional, from clamped in-range addresses (the last row
   // for a team past the end, entry 0 for positions the 2/3 rule leaves
-  // empty), and the unused values are zeroed by selects afterwards: loads
-  // under lane-dependent branches each end in a wait for memory, which
-  // serialises a row's dozens of loads.
+  // empty), and the unused values are zeroed afterwards: loads under
+  // lane-dependent branches each end in a wait for memory, which serialises
+  // a row's dozens of loads.  The loads of pair p + 1 are issued before the
+  // transform of pair p (two pairs in flight per wavefront).
   const long rl = live ? r : nrows - 1;
   T q[NIN][E];
+  cpx<T> la[2][E], lb[2][E];
+  row_loads<T, N, PL, NIN>(la[0], lb[0], a, 0, rl, lane, cx);
 #pragma unroll
   for (int p = 0; p < NPI; ++p) {
+    if (p + 1 < NPI) row_loads<T, N, PL, NIN>(la[(p + 1) & 1], lb[(p + 1) & 1], a, p + 1, rl, lane, cx);
     cpx<T> v[E];
     const bool hasb = (2 * p + 1 < NIN);
-    const cpx<T>* __restrict__ ia = a.in[2 * p] + rl * a.kxp;
-    const cpx<T>* __restrict__ ib = a.in[hasb ? 2 * p + 1 : 2 * p] + rl * a.kxp;
-    cpx<T> la[E], lb[E];
-#pragma unroll
-    for (int qq = 0; qq < E / R0; ++qq)
-#pragma unroll
-      for (int rr = 0; rr < R0; ++rr) {
-        const int m = in_pos<N, LANES, R0>(lane, qq, rr);
-        const int idx = m <= cx ? m : (m >= N - cx ? N - m : 0);
-        la[qq * R0 + rr] = ia[idx];
-        lb[qq * R0 + rr] = hasb ? ib[idx] : mk(T(0), T(0));
-      }
 #pragma unroll
     for (int qq = 0; qq < E / R0; ++qq)
 #pragma unroll
@@ -250,10 +262,13 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
         const int m = in_pos<N, LANES, R0>(lane, qq, rr);
         const bool keep = live && (m <= cx || m >= N - cx);
         const bool mirror = m >= N - cx;
-        const cpx<T> va = la[qq * R0 + rr], vb = lb[qq * R0 + rr];
-        // conj for mirrored positions; the k = 0 entry is real
-        const T ay = m == 0 ? T(0) : (mirror ? -va.y : va.y);
-        const T by = m == 0 ? T(0) : (mirror ? -vb.y : vb.y);
+        const cpx<T> va = la[p & 1][qq * R0 + rr], vb = lb[p & 1][qq * R0 + rr];
+        // conj for mirrored positions; the k = 0 entry is taken as real.  The
+        // k = 0 imaginary parts are scaled by 0 rather than selected away, so
+        // that their loads stay unconditional.
+        const T z0 = m == 0 ? T(0) : T(1);
+        const T ay = (mirror ? -va.y : va.y) * z0;
+        const T by = (mirror ? -vb.y : vb.y) * z0;
         v[qq * R0 + rr] = keep ? mk(va.x - by, ay + vb.x) : mk(T(0), T(0));
       }
     run_passes_cached<T, 1, N, PL, false, 1, POW>(v, lane, 0, buf, twi);
