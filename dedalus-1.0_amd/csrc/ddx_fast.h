@@ -421,8 +421,24 @@ col_fft_fast(const ColArgs<T> a) {
   constexpr int BUF = exch_size<N, PL>();
   __shared__ cpx<T> lds[BUF * W];
   const int c = threadIdx.x % W, lane = threadIdx.x / W;
-  const int f = blockIdx.z, tile = blockIdx.x;
-  const long slab = blockIdx.y;
+  int f, tile;
+  long slab;
+  if (a.xcd_group) {
+    // Workgroups are dealt to the 8 XCDs round robin by linear index L, so
+    // L = 8 (floor(U / 8) F + f) + U mod 8 puts the F fields of unit
+    // U = (slab, tile) on one XCD, one after another: fields that read the
+    // same sources (a spectral prologue) then find them in that XCD's L2.
+    const long L = blockIdx.x, t = L >> 3;
+    const long U = (t / a.nfields) * 8 + (L & 7);
+    if (U >= (long)a.ntiles * a.nslabs) return;   // padding of the last group
+    f = (int)(t % a.nfields);
+    tile = (int)(U % a.ntiles);
+    slab = U / a.ntiles;
+  } else {
+    f = blockIdx.z;
+    tile = blockIdx.x;
+    slab = blockIdx.y;
+  }
   const int col = tile * W + c;
   const bool live = col < a.ncols;
   cpx<T> v[E];

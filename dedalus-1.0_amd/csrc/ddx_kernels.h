@@ -50,6 +50,8 @@ struct ColArgs {
   int W;                 // tile width (columns per workgroup)
   int row_dim, slab_dim; // physical axis (0 x, 1 y, 2 z) of rows / slabs, -1 none
   int nslabs, nfields;   // grid extents (used by the remapped 1D grids)
+  int ntiles;            // column tiles per slab
+  int xcd_group;         // 1D grid: the fields of one (tile, slab) on one XCD
   T scale;
 };
 
