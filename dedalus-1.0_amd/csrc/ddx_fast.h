@@ -39,6 +39,14 @@ template <> struct ColPlan<512> { static constexpr int LANES = 64, E = 8,  W = 4
 template <int P, int R>
 __device__ __forceinline__ int xpad(int m) { return m + (m / (P * R)) * P; }
 
+// The exchange indices are affine in the leg r, so each thread computes one
+// base and the legs use constant offsets (immediate LDS offsets): with
+// k = b mod P, xpad(out_pos(b, r)) = xpad(out_pos(b, 0)) + r P because
+// k + r P < P R; and with S = N / R2 a multiple of P R,
+// xpad(in_pos(b, r)) = xpad(in_pos(b, 0)) + r (S + S / R).
+template <int N, int P, int R, int R2>
+constexpr int RSTEP() { return N / R2 + N / R2 / R; }
+
 // LDS entries per transform needed by the padded exchanges of a plan
 template <int N, class PL>
 constexpr int exch_size() {
@@ -117,16 +125,18 @@ __device__ __forceinline__ void run_passes(cpx<T> (&v)[PL::E], int lane, int c, 
   if constexpr (I + 1 < NP) {
     constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
 #pragma unroll
-    for (int q = 0; q < PL::E / R; ++q)
+    for (int q = 0; q < PL::E / R; ++q) {
+      const int wb = xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, 0)) * WS + c;
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        buf[xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, r)) * WS + c] = v[q * R + r];
+      for (int r = 0; r < R; ++r) buf[wb + r * (P * WS)] = v[q * R + r];
+    }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PL::E / R2; ++q)
+    for (int q = 0; q < PL::E / R2; ++q) {
+      const int rb = xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, 0)) * WS + c;
 #pragma unroll
-      for (int r = 0; r < R2; ++r)
-        v[q * R2 + r] = buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c];
+      for (int r = 0; r < R2; ++r) v[q * R2 + r] = buf[rb + r * (RSTEP<N, P, R, R2>() * WS)];
+    }
     __syncthreads();
     run_passes<T, D, N, PL, REV, WS, I + 1, P * R>(v, lane, c, buf, tw);
   }
@@ -164,16 +174,18 @@ __device__ __forceinline__ void run_passes_cached(cpx<T> (&v)[PL::E], int lane, 
   if constexpr (I + 1 < NP) {
     constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
 #pragma unroll
-    for (int q = 0; q < PL::E / R; ++q)
+    for (int q = 0; q < PL::E / R; ++q) {
+      const int wb = xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, 0)) * WS + c;
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        buf[xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, r)) * WS + c] = v[q * R + r];
+      for (int r = 0; r < R; ++r) buf[wb + r * (P * WS)] = v[q * R + r];
+    }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PL::E / R2; ++q)
+    for (int q = 0; q < PL::E / R2; ++q) {
+      const int rb = xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, 0)) * WS + c;
 #pragma unroll
-      for (int r = 0; r < R2; ++r)
-        v[q * R2 + r] = buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c];
+      for (int r = 0; r < R2; ++r) v[q * R2 + r] = buf[rb + r * (RSTEP<N, P, R, R2>() * WS)];
+    }
     __syncthreads();
     run_passes_cached<T, D, N, PL, REV, WS, POW, I + 1, P * R>(v, lane, c, buf, twc);
   }
@@ -192,28 +204,33 @@ __device__ __forceinline__ void run_passes_split(cpx<T> (&v)[PL::E], int lane, i
     constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
     T nx[PL::E];
 #pragma unroll
-    for (int q = 0; q < PL::E / R; ++q)
+    for (int q = 0; q < PL::E / R; ++q) {
+      const int wb = xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, 0)) * WS + c;
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        buf[xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, r)) * WS + c] = v[q * R + r].x;
+      for (int r = 0; r < R; ++r) buf[wb + r * (P * WS)] = v[q * R + r].x;
+    }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PL::E / R2; ++q)
+    for (int q = 0; q < PL::E / R2; ++q) {
+      const int rb = xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, 0)) * WS + c;
+#pragma unroll
+      for (int r = 0; r < R2; ++r) nx[q * R2 + r] = buf[rb + r * (RSTEP<N, P, R, R2>() * WS)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PL::E / R; ++q) {
+      const int wb = xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, 0)) * WS + c;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[wb + r * (P * WS)] = v[q * R + r].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PL::E / R2; ++q) {
+      const int rb = xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, 0)) * WS + c;
 #pragma unroll
       for (int r = 0; r < R2; ++r)
-        nx[q * R2 + r] = buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PL::E / R; ++q)
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        buf[xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, r)) * WS + c] = v[q * R + r].y;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PL::E / R2; ++q)
-#pragma unroll
-      for (int r = 0; r < R2; ++r)
-        v[q * R2 + r] = mk(nx[q * R2 + r], buf[xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, r)) * WS + c]);
+        v[q * R2 + r] = mk(nx[q * R2 + r], buf[rb + r * (RSTEP<N, P, R, R2>() * WS)]);
+    }
     __syncthreads();
     run_passes_split<T, D, N, PL, REV, WS, POW, I + 1, P * R>(v, lane, c, buf, twc);
   }
@@ -274,7 +291,12 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
   constexpr int NPI = (NIN + 1) / 2, NPO = (NOUT + 1) / 2;
   constexpr int R0 = PL::R[0];
   constexpr int BUF = exch_size<N, PL>();
+  // FP64 rows of 12 fields park the x-space values of the first two input
+  // pairs in LDS until the products (8 KB per row at n = 256), which keeps
+  // the kernel within the registers of 3 wavefronts per SIMD
+  constexpr int NPARK = (NIN >= 12 && sizeof(T) == 8) ? 4 : 0;
   __shared__ cpx<T> lds[RPB][BUF];
+  __shared__ T park[RPB][NPARK > 0 ? NPARK : 1][NPARK > 0 ? N : 1];
   const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
   const long r = (long)blockIdx.x * RPB + team;
   const bool live = r < nrows;
@@ -322,11 +344,21 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
     run_passes_cached<T, 1, N, PL, false, 1, POW>(v, lane, 0, buf, twi);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      q[2 * p][e] = v[e].x;
-      if (hasb) q[2 * p + 1][e] = v[e].y;
+      if (2 * p + 1 < NPARK) {
+        // each lane reads back only what it wrote: no synchronisation needed
+        park[team][2 * p][e * LANES + lane] = v[e].x;
+        park[team][2 * p + 1][e * LANES + lane] = v[e].y;
+      } else {
+        q[2 * p][e] = v[e].x;
+        if (hasb) q[2 * p + 1][e] = v[e].y;
+      }
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int f = 0; f < NPARK; ++f)
+#pragma unroll
+    for (int e = 0; e < E; ++e) q[f][e] = park[team][f][e * LANES + lane];
 
   T o[NOUT][E];
 #pragma unroll
