@@ -119,19 +119,43 @@ class Ensemble:
             for g in self.groups:
                 g.step(self.params, 1, self.dt)
 
-    def capture(self):
-        """One graph holding one RK2mid step of every group."""
+    def capture(self, concurrent=True):
+        """One graph holding one RK2mid step of every group.
+
+        concurrent: the groups are independent, so each group other than the
+        first is captured on a side stream forked from and joined back to the
+        ensemble's stream; the graph then lets the small groups' launches,
+        which do not fill the GPU, run beside the large group's."""
         import torch
         for g in self.groups:
             g.plan.set_graph(False)
+        side = [torch.cuda.Stream() for _ in self.groups[1:]] if concurrent else []
         with torch.cuda.stream(self.stream):
             self.step_eager(1)                       # workspaces, twiddles
             self.stream.synchronize()
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph, stream=self.stream):
-                self.step_eager(1)
+                if not side:
+                    self.step_eager(1)
+                else:
+                    fork = torch.cuda.Event()
+                    fork.record(self.stream)
+                    joins = []
+                    for g, s in zip(self.groups[1:], side):
+                        s.wait_event(fork)
+                        g.plan.set_stream(s.cuda_stream)
+                        g.step(self.params, 1, self.dt)
+                        e = torch.cuda.Event()
+                        e.record(s)
+                        joins.append(e)
+                    self.groups[0].step(self.params, 1, self.dt)
+                    for e in joins:
+                        self.stream.wait_event(e)
+        for g in self.groups[1:]:
+            g.plan.set_stream(self.stream.cuda_stream)
         self.stream.synchronize()
         self.graph = graph
+        self._side = side                            # kept alive with the graph
         return graph
 
     def advance(self, nsteps):
