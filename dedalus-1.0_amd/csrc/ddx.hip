@@ -237,9 +237,10 @@ enum ColKind { COL_PLAIN, COL_SPECTRAL };
 template <typename T, int N>
 void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
   using PL = ColPlan<N>;
-  const int ntiles = (a.ncols + PL::W - 1) / PL::W;
+  constexpr int W = col_width<T, N>();
+  const int ntiles = (a.ncols + W - 1) / W;
   dim3 grid(ntiles, nslabs, nfields);
-  dim3 block(PL::LANES * PL::W);
+  dim3 block(PL::LANES * W);
   a.nslabs = nslabs;
   a.nfields = nfields;
   a.ntiles = ntiles;

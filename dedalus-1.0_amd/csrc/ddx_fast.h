@@ -32,6 +32,11 @@ template <> struct ColPlan<128> { static constexpr int LANES = 8,  E = 16, W = 1
 template <> struct ColPlan<256> { static constexpr int LANES = 16, E = 16, W = 8,  NP = 2; static constexpr int R[2] = {16, 16}; };
 template <> struct ColPlan<512> { static constexpr int LANES = 64, E = 8,  W = 4,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
 
+// Column tile width of the kernel for element type T: FP32 tiles at n = 256
+// are twice as wide, so a tile row is a whole 128-byte line as in FP64.
+template <typename T, int N>
+constexpr int col_width() { return ColPlan<N>::W * ((sizeof(T) == 4 && N == 256) ? 2 : 1); }
+
 // LDS index of position m in the exchange written by a pass of radix R after
 // P points: runs of P contiguous outputs start every P*R positions; shifting
 // run t by t*P (R + 1 odd) spreads the runs of consecutive threads over
@@ -492,10 +497,10 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
 }
 
 template <typename T, int D, int N, bool SPECTRAL>
-__global__ void __launch_bounds__(ColPlan<N>::LANES * ColPlan<N>::W)
+__global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<T, N>()))
 col_fft_fast(const ColArgs<T> a) {
   using PL = ColPlan<N>;
-  constexpr int LANES = PL::LANES, E = PL::E, W = PL::W;
+  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<T, N>();
   constexpr int R0 = PL::R[0];
   constexpr int RL = PL::R[PL::NP - 1];
   constexpr int BUF = exch_size<N, PL>();
