@@ -52,6 +52,17 @@ __device__ __forceinline__ int xpad(int m) { return m + (m / (P * R)) * P; }
 template <int N, int P, int R, int R2>
 constexpr int RSTEP() { return N / R2 + N / R2 / R; }
 
+// The first exchange of a row transform (P = 1, radix <= 4, one transform
+// per buffer) writes positions R b + r and reads lane + S r.  The padding
+// above leaves the reads with two-way bank conflicts; this XOR swizzle of
+// bits 0-1 by bits 3-4 is a permutation of [0, n) under which both the
+// writes and the reads of 8 (or 16) consecutive lanes fall in distinct
+// 16-byte bank groups.
+__device__ __forceinline__ int xswz(int m) { return m ^ ((m >> 3) & 3) ^ ((m >> 4) & 3); }
+
+template <int N, int WS, int P, int R>
+constexpr bool swizzled() { return WS == 1 && P == 1 && R <= 4 && N >= 64 && N <= 256; }
+
 // LDS entries per transform needed by the padded exchanges of a plan
 template <int N, class PL>
 constexpr int exch_size() {
@@ -178,18 +189,30 @@ __device__ __forceinline__ void run_passes_cached(cpx<T> (&v)[PL::E], int lane, 
   reg_pass_cached<T, D, PL::E, R, P, POW>(v, twc[I]);
   if constexpr (I + 1 < NP) {
     constexpr int R2 = PL::R[REV ? NP - 2 - I : I + 1];
+    if constexpr (swizzled<N, WS, P, R>()) {
 #pragma unroll
-    for (int q = 0; q < PL::E / R; ++q) {
-      const int wb = xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, 0)) * WS + c;
+      for (int q = 0; q < PL::E / R; ++q)
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[wb + r * (P * WS)] = v[q * R + r];
-    }
-    __syncthreads();
+        for (int r = 0; r < R; ++r) buf[xswz(out_pos<N, PL::LANES, R, P>(lane, q, r))] = v[q * R + r];
+      __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PL::E / R2; ++q) {
-      const int rb = xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, 0)) * WS + c;
+      for (int q = 0; q < PL::E / R2; ++q)
 #pragma unroll
-      for (int r = 0; r < R2; ++r) v[q * R2 + r] = buf[rb + r * (RSTEP<N, P, R, R2>() * WS)];
+        for (int r = 0; r < R2; ++r) v[q * R2 + r] = buf[xswz(in_pos<N, PL::LANES, R2>(lane, q, r))];
+    } else {
+#pragma unroll
+      for (int q = 0; q < PL::E / R; ++q) {
+        const int wb = xpad<P, R>(out_pos<N, PL::LANES, R, P>(lane, q, 0)) * WS + c;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[wb + r * (P * WS)] = v[q * R + r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PL::E / R2; ++q) {
+        const int rb = xpad<P, R>(in_pos<N, PL::LANES, R2>(lane, q, 0)) * WS + c;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[q * R2 + r] = buf[rb + r * (RSTEP<N, P, R, R2>() * WS)];
+      }
     }
     __syncthreads();
     run_passes_cached<T, D, N, PL, REV, WS, POW, I + 1, P * R>(v, lane, c, buf, twc);
