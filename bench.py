@@ -20,6 +20,10 @@ Fields of the JSON line besides the standard ones:
                 one exists for this kernel and workload, else null
   step_roofline the whole step against the byte model of B_alg per
                 point-step (compulsory traffic of the minimal dataflow)
+  hbm_measured  the metric's %HBM: HBM bytes per step measured by
+                rocprofv3 FETCH_SIZE + WRITE_SIZE (the step summary in
+                profiles/ for this workload and precision) x K / wall of
+                the timed region, against 8 TB/s; null without a summary
   cpu_baseline  the numpy oracle (a restatement of the reference's numpy
                 path) timed on this host for one step of the same workload
   closure       (default workload) the metric's second part: the learned
@@ -257,6 +261,19 @@ def closure_summary(batches=(1, 64, 1024), iters=None, t_eval=10.0):
     }
 
 
+def step_traffic_from_profiles(workload, precision):
+    """HBM bytes per step from the committed rocprofv3 step summary, if any."""
+    path = os.path.join(ROOT, "profiles", "step_%s_hbm_traffic.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    key = "hbm_bytes_per_step_%s" % precision
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                return float(line.split("=", 1)[1])
+    return None
+
+
 def traffic_from_profiles(kernel, workload):
     """HBM bytes per launch from a committed rocprofv3 summary, if present."""
     path = os.path.join(ROOT, "profiles", "%s_%s_pmc.txt" % (kernel, workload))
@@ -370,6 +387,13 @@ def main():
         step_roof = {"bytes_per_point_step": balg, "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
+    hbm_meas = None
+    step_bytes = step_traffic_from_profiles(w["name"], args.precision)
+    if step_bytes:
+        gbs = step_bytes * args.steps / wall / 1e9
+        hbm_meas = {"bytes_per_step": step_bytes, "achieved": round(gbs, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -391,6 +415,7 @@ def main():
                    "params": w["params"]},
         "roofline": roof,
         "step_roofline": step_roof,
+        "hbm_measured": hbm_meas,
         "kernels": kernels,
     }
     if not args.no_closure and w["name"] == "mhd3d_256":
