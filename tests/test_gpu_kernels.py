@@ -186,6 +186,18 @@ RHS_CASES = [
     ('mhd', (32, 32), {'rho0': 1.3}),
     ('mhd', (16, 16, 16), {'rho0': 0.8}),
     ('mhd', (8, 12, 16), {}),
+    # size-specialised 3D kernels (every extent in 64..512): the x-row kernels
+    # of 6, 10 and 12 fields, the spectral z pass and the z / y column passes
+    ('hydro', (64, 64, 64), {'nu': 1e-3}),
+    ('boussinesq', (64, 64, 64), {'boussinesq_direction': 'z', 'g': 1.1, 'alpha_t': 0.9, 'beta': 0.8}),
+    ('boussinesq', (64, 128, 64), {'boussinesq_direction': 'x'}),
+    ('mhd', (64, 64, 64), {'rho0': 1.2}),
+    ('mhd', (128, 64, 64), {}),
+    ('mhd', (64, 64, 128), {'rho0': 0.9}),
+    # the n = 256 kernels of the 256^3 MHD workload, one axis at a time
+    ('mhd', (64, 64, 256), {}),
+    ('mhd', (64, 256, 64), {'rho0': 1.1}),
+    ('mhd', (256, 64, 64), {}),
 ]
 
 
@@ -215,6 +227,10 @@ STEP_CASES = [
     ('mhd', (32, 32), {'nu': 1e-3, 'eta': 2e-3}, 'RK2mid', 1e-2, 10),
     ('mhd', (16, 16, 16), {'nu': 1e-3, 'eta': 1e-3}, 'RK2mid', 5e-3, 5),
     ('mhd', (16, 16, 16), {'nu': 1e-3, 'eta': 1e-3}, 'RK2trap', 5e-3, 5),
+    ('hydro', (64, 64, 64), {'nu': 2e-3}, 'RK2trap', 5e-3, 3),
+    ('boussinesq', (64, 64, 64), {'boussinesq_direction': 'y', 'nu': 1e-3, 'kappa': 2e-3}, 'RK2trap', 5e-3, 3),
+    ('mhd', (64, 64, 64), {'nu': 1e-3, 'eta': 2e-3}, 'RK2mid', 5e-3, 3),
+    ('mhd', (64, 64, 64), {'nu': 1e-3, 'eta': 1e-3, 'viscosity_order': 2}, 'RK2trap', 2e-3, 3),
 ]
 
 
@@ -299,7 +315,8 @@ def test_steps_with_constant_forcing(shape, integ):
 
 FUSE_CASES = [('hydro', (32, 32), 'RK2mid'), ('hydro', (16, 16, 16), 'RK2trap'),
               ('boussinesq', (32, 48), 'RK2mid'), ('boussinesq', (16, 8, 16), 'RK2trap'),
-              ('mhd', (16, 16, 16), 'RK2mid'), ('mhd', (32, 32), 'RK2trap')]
+              ('mhd', (16, 16, 16), 'RK2mid'), ('mhd', (32, 32), 'RK2trap'),
+              ('boussinesq', (64, 64, 64), 'RK2trap'), ('mhd', (64, 64, 64), 'RK2mid')]
 
 
 @pytest.mark.parametrize("kind,shape,integ", FUSE_CASES, ids=str)
