@@ -28,6 +28,16 @@ int set_error(int code, const std::string& msg);  // defined in ddx.hip
 
 namespace {
 
+// Workgroup barrier that orders LDS only: the release/acquire fences name the
+// local address space, so global loads (the next tile's halo, issued before
+// the current tile's compute) and stores stay in flight across it; a plain
+// __syncthreads() also waits for every outstanding global access.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -222,10 +232,17 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
   };
   const int wave = tid >> 6, lane = tid & 63;
   const int m = lane & 15, kq = lane >> 4;
+  const bool grad = a.mode == DDX_CONV_GELU_GRAD, gelu_on = a.mode == DDX_CONV_GELU;
+  float bias4[4] = {0.f, 0.f, 0.f, 0.f};
+  if (!grad && a.bias) {
+    const int cb = (lane % (CO / 4)) * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) bias4[e] = a.bias[cb + e];
+  }
   long tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   for (; tile < ntiles; tile += gridDim.x) {
-    __syncthreads();  // the previous tile's fragment reads are done
+    lds_barrier();  // the previous tile's fragment reads are done
 #pragma unroll
     for (int k = 0; k < HPT; ++k) {
       const int i = tid + k * 256;
@@ -236,7 +253,7 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
         d[1] = make_uint2(hv[k].z, hv[k].w);
       }
     }
-    __syncthreads();
+    lds_barrier();
     const long next = tile + gridDim.x;
     if (next < ntiles) fetch(next);  // in flight while this tile computes
     f32x4 acc[4][NCO];
@@ -284,7 +301,7 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
     const int b = (int)(rest / tiles_y);
     const long prow = ((long)b * a.H + ty * 4 + wave) * a.W + tx * 64;
     // stage D fragments (pixel q*16 + 4kq + i, channel j*16 + m) as [64][ES] fp32
-    __syncthreads();  // every wavefront is done with the halo
+    lds_barrier();  // every wavefront is done with the halo
     float* st = reinterpret_cast<float*>(halo) + wave * 64 * ES;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -292,19 +309,29 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
       for (int j = 0; j < NCO; ++j)
 #pragma unroll
         for (int i = 0; i < 4; ++i) st[(q * 16 + 4 * kq + i) * ES + j * 16 + m] = acc[q][j][i];
-    __syncthreads();
-    // the wavefront's 64 pixels x CO channels are contiguous in HBM: 4 channels per chunk
-    constexpr int C4 = CO / 4;
+    lds_barrier();
+    // the wavefront's 64 pixels x CO channels are contiguous in HBM: 4 channels
+    // per chunk; a lane's channels c..c+3 are the same in every chunk (C4
+    // divides 64), so its bias was read once before the tile loop.  The aux
+    // loads of all chunks are issued before any is used.
+    constexpr int C4 = CO / 4, NR = (64 * C4) / 64;
+    const int c = (lane % C4) * 4;
+    float4 zaux[NR];
+    if (grad) {
 #pragma unroll
-    for (int r = 0; r < (64 * C4) / 64; ++r) {
-      const int id = lane + 64 * r;
-      const int px = id / C4, c = (id - px * C4) * 4;
+      for (int r = 0; r < NR; ++r) {
+        const int px = (lane + 64 * r) / C4;
+        zaux[r] = *reinterpret_cast<const float4*>(&a.aux[(prow + px) * CO + c]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int px = (lane + 64 * r) / C4;
       const float4 d = *reinterpret_cast<const float4*>(&st[px * ES + c]);
       float v[4] = {d.x, d.y, d.z, d.w}, act[4];
       const long o = (prow + px) * CO + c;
-      if (a.mode == DDX_CONV_GELU_GRAD) {
-        const float4 z = *reinterpret_cast<const float4*>(&a.aux[o]);
-        const float zz[4] = {z.x, z.y, z.z, z.w};
+      if (grad) {
+        const float zz[4] = {zaux[r].x, zaux[r].y, zaux[r].z, zaux[r].w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           v[e] *= gelu_grad(zz[e]);
@@ -313,8 +340,8 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          v[e] += a.bias ? a.bias[c + e] : 0.f;
-          act[e] = a.mode == DDX_CONV_GELU ? gelu(v[e]) : v[e];
+          v[e] += bias4[e];
+          act[e] = gelu_on ? gelu(v[e]) : v[e];
         }
       }
       if (a.out_f32) *reinterpret_cast<float4*>(&a.out_f32[o]) = make_float4(v[0], v[1], v[2], v[3]);
@@ -413,7 +440,7 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t*
     const int ty = (int)(rest % tiles_y);
     const int b = (int)(rest / tiles_y);
     const int x0 = tx * 64, y0 = ty * 4;
-    __syncthreads();  // the previous tile's reads are done
+    lds_barrier();  // the previous tile's reads are done
     // dz: 256 pixels x CO, 4 channels per load, stored transposed
     for (int i = tid; i < 256 * (CO / 4); i += 256) {
       const int pix = i / (CO / 4), ch = (i - pix * (CO / 4)) * 4;
@@ -438,7 +465,7 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t*
       haloT[(ch + 2) * HS + o] = (uint16_t)(v.y & 0xffffu);
       haloT[(ch + 3) * HS + o] = (uint16_t)(v.y >> 16);
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int u = 0; u < NPW; ++u) {
       const int pair = wave + 4 * u;
