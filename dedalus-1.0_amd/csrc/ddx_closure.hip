@@ -177,7 +177,12 @@ __device__ inline bf16x8 lds8(const uint16_t* p) {
 // tile computes.  Pixel stride and weight-row stride CI + 4 (bf16) place the
 // 16 rows of a fragment read in distinct bank pairs.  CI = 32 uses the
 // gfx950 16x16x32 bf16 MFMA, CI = 16 the 16x16x16 one.
-template <int CI, int CO>
+//
+// BF (bf16 output only, no fp32 output or GELU gradient: the inference
+// pass): bias and activation are applied to the accumulators and the
+// results staged as bf16, which halves the staging LDS; the kernel then fits
+// three workgroups per CU instead of two.
+template <int CI, int CO, bool BF>
 __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntiles) {
   constexpr int PS = CI + 4, WS = CI + 4;
   constexpr int HR = 6, HC = 66;
@@ -185,9 +190,11 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
   constexpr int CK = CI / 8;                       // 16-byte chunks per pixel
   constexpr int NH = HR * HC * CK, HPT = (NH + 255) / 256;
   constexpr int NW = CO * 9 * CK, WPT = (NW + 255) / 256;
-  // the halo buffer also stages each wavefront's 64 x CO fp32 results
-  constexpr int ES = CO + 4;                                  // staging row stride (floats)
-  constexpr int HALO_B = HR * HC * PS * 2, STAGE_B = 4 * 64 * ES * 4;
+  // the halo buffer also stages each wavefront's 64 x CO results (fp32, or
+  // bf16 with BF)
+  constexpr int ES = CO + 4;                                  // fp32 staging row stride (floats)
+  constexpr int EB = CO + 8;                                  // bf16 staging row stride (bf16)
+  constexpr int HALO_B = HR * HC * PS * 2, STAGE_B = BF ? 4 * 64 * EB * 2 : 4 * 64 * ES * 4;
   constexpr int BUF_B = HALO_B > STAGE_B ? HALO_B : STAGE_B;
   __shared__ __align__(16) uint16_t halo[BUF_B / 2];
   __shared__ __align__(16) uint16_t wl[CO * 9 * WS];
@@ -233,11 +240,16 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
   const int wave = tid >> 6, lane = tid & 63;
   const int m = lane & 15, kq = lane >> 4;
   const bool grad = a.mode == DDX_CONV_GELU_GRAD, gelu_on = a.mode == DDX_CONV_GELU;
-  float bias4[4] = {0.f, 0.f, 0.f, 0.f};
+  float bias4[4] = {0.f, 0.f, 0.f, 0.f};   // channels (lane % (CO/4)) * 4 + e (fp32 staging)
+  float biasj[NCO];                         // channels j * 16 + m (BF)
+#pragma unroll
+  for (int j = 0; j < NCO; ++j) biasj[j] = 0.f;
   if (!grad && a.bias) {
     const int cb = (lane % (CO / 4)) * 4;
 #pragma unroll
     for (int e = 0; e < 4; ++e) bias4[e] = a.bias[cb + e];
+#pragma unroll
+    for (int j = 0; j < NCO; ++j) biasj[j] = a.bias[j * 16 + m];
   }
   long tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
@@ -300,8 +312,32 @@ __global__ void __launch_bounds__(256) conv3x3_tile_kernel(ConvArgs a, long ntil
     const int ty = (int)(rest % tiles_y);
     const int b = (int)(rest / tiles_y);
     const long prow = ((long)b * a.H + ty * 4 + wave) * a.W + tx * 64;
-    // stage D fragments (pixel q*16 + 4kq + i, channel j*16 + m) as [64][ES] fp32
     lds_barrier();  // every wavefront is done with the halo
+    if constexpr (BF) {
+      // activations of the D fragments (pixel q*16 + 4kq + i, channel j*16 + m)
+      // as [64][EB] bf16, then 4 channels (8 bytes) per lane and chunk to HBM
+      uint16_t* sb = halo + wave * 64 * EB;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < NCO; ++j)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float v = acc[q][j][i] + biasj[j];
+            sb[(q * 16 + 4 * kq + i) * EB + j * 16 + m] = f2bf(gelu_on ? gelu(v) : v);
+          }
+      lds_barrier();
+      constexpr int C4 = CO / 4, NR = (64 * C4) / 64;
+      const int c = (lane % C4) * 4;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int px = (lane + 64 * r) / C4;
+        *reinterpret_cast<uint2*>(&a.out_bf16[(prow + px) * CO + c]) =
+            *reinterpret_cast<const uint2*>(&sb[px * EB + c]);
+      }
+      continue;
+    }
+    // stage D fragments (pixel q*16 + 4kq + i, channel j*16 + m) as [64][ES] fp32
     float* st = reinterpret_cast<float*>(halo) + wave * 64 * ES;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -731,10 +767,18 @@ int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x
     const long ntiles = (long)B * (H / 4) * (W / 64);
     const dim3 tgrid((unsigned)(ntiles < 1024 ? ntiles : 1024));
     const int key = Ci * 100 + Co;
-    if (key == 1616) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 16>), tgrid, dim3(256), 0, s, a, ntiles);
-    else if (key == 1632) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 32>), tgrid, dim3(256), 0, s, a, ntiles);
-    else if (key == 3216) hipLaunchKernelGGL((conv3x3_tile_kernel<32, 16>), tgrid, dim3(256), 0, s, a, ntiles);
-    else hipLaunchKernelGGL((conv3x3_tile_kernel<32, 32>), tgrid, dim3(256), 0, s, a, ntiles);
+    // bf16-only output without the GELU gradient stages bf16 (BF)
+    if (!out_f32 && mode != DDX_CONV_GELU_GRAD) {
+      if (key == 1616) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 16, true>), tgrid, dim3(256), 0, s, a, ntiles);
+      else if (key == 1632) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 32, true>), tgrid, dim3(256), 0, s, a, ntiles);
+      else if (key == 3216) hipLaunchKernelGGL((conv3x3_tile_kernel<32, 16, true>), tgrid, dim3(256), 0, s, a, ntiles);
+      else hipLaunchKernelGGL((conv3x3_tile_kernel<32, 32, true>), tgrid, dim3(256), 0, s, a, ntiles);
+    } else {
+      if (key == 1616) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 16, false>), tgrid, dim3(256), 0, s, a, ntiles);
+      else if (key == 1632) hipLaunchKernelGGL((conv3x3_tile_kernel<16, 32, false>), tgrid, dim3(256), 0, s, a, ntiles);
+      else if (key == 3216) hipLaunchKernelGGL((conv3x3_tile_kernel<32, 16, false>), tgrid, dim3(256), 0, s, a, ntiles);
+      else hipLaunchKernelGGL((conv3x3_tile_kernel<32, 32, false>), tgrid, dim3(256), 0, s, a, ntiles);
+    }
     CL_LAUNCHCHK("conv3x3 tile");
     return 0;
   }
