@@ -237,7 +237,7 @@ enum ColKind { COL_PLAIN, COL_SPECTRAL };
 template <typename T, int N>
 void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
   using PL = ColPlan<N>;
-  constexpr int W = col_width<T, N>();
+  const int W = spectral ? col_width<T, N, true>() : col_width<T, N, false>();
   const int ntiles = (a.ncols + W - 1) / W;
   dim3 grid(ntiles, nslabs, nfields);
   dim3 block(PL::LANES * W);

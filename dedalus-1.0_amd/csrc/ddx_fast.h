@@ -32,10 +32,15 @@ template <> struct ColPlan<128> { static constexpr int LANES = 8,  E = 16, W = 1
 template <> struct ColPlan<256> { static constexpr int LANES = 16, E = 16, W = 8,  NP = 2; static constexpr int R[2] = {16, 16}; };
 template <> struct ColPlan<512> { static constexpr int LANES = 64, E = 8,  W = 4,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
 
-// Column tile width of the kernel for element type T: FP32 tiles at n = 256
-// are twice as wide, so a tile row is a whole 128-byte line as in FP64.
-template <typename T, int N>
-constexpr int col_width() { return ColPlan<N>::W * ((sizeof(T) == 4 && N == 256) ? 2 : 1); }
+// Column tile width of the kernel for element type T: at n = 256, FP32 tiles
+// are 16 columns wide, so a tile row is a whole 128-byte line, and FP64
+// tiles of the plain passes too (256-byte rows; their exchanges then go
+// through the half-size real buffer): 6 % faster y passes at 256^3.  The
+// spectral pass keeps 8 FP64 columns, where 16 were slower.
+template <typename T, int N, bool SPECTRAL>
+constexpr int col_width() {
+  return ColPlan<N>::W * ((N == 256 && (sizeof(T) == 4 || !SPECTRAL)) ? 2 : 1);
+}
 
 // LDS index of position m in the exchange written by a pass of radix R after
 // P points: runs of P contiguous outputs start every P*R positions; shifting
@@ -520,17 +525,19 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
 }
 
 template <typename T, int D, int N, bool SPECTRAL>
-__global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<T, N>()))
+__global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<T, N, SPECTRAL>()))
 col_fft_fast(const ColArgs<T> a) {
   using PL = ColPlan<N>;
-  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<T, N>();
+  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<T, N, SPECTRAL>();
   constexpr int R0 = PL::R[0];
   constexpr int RL = PL::R[PL::NP - 1];
   constexpr int BUF = exch_size<N, PL>();
   // FP64 exchanges for n <= 128 go through a buffer of reals in two halves,
-  // which lifts the occupancy LDS allowed (8 % faster passes at 128^3; at
-  // n = 256 the extra exchange round cost more than the occupancy gained)
-  constexpr bool SPLIT = sizeof(T) == 8 && N <= 128;
+  // which lifts the occupancy LDS allowed (8 % faster passes at 128^3); at
+  // n = 256 only the 16-column tiles use it (their complex buffer would
+  // exceed 64 KB); for 8-column tiles the extra round cost more than the
+  // occupancy gained
+  constexpr bool SPLIT = sizeof(T) == 8 && (N <= 128 || W == 16);
   __shared__ typename std::conditional<SPLIT, T, cpx<T>>::type lds[BUF * W];
   const int c = threadIdx.x % W, lane = threadIdx.x / W;
   int f, tile;
