@@ -403,6 +403,30 @@ bool launch_row_fast(ddx_plan* p, const RowArgs<T>& a, long nrows) {
   }
 }
 
+// Size-specialised single-field c2r / r2c row passes (two rows per team);
+// false if none applies.
+template <typename T, int N>
+void row_plain_n(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  constexpr int RPB = 64 / RowPlan<N>::LANES;
+  const long teams = (nrows + 1) / 2;
+  const dim3 grid((unsigned)((teams + RPB - 1) / RPB));
+  if (a.mode == ROW_C2R) hipLaunchKernelGGL((row_c2r_fast<T, N>), grid, dim3(64), 0, p->stream, a, nrows);
+  else hipLaunchKernelGGL((row_r2c_fast<T, N>), grid, dim3(64), 0, p->stream, a, nrows);
+}
+
+template <typename T>
+bool launch_row_plain_fast(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  if (a.kx > a.nx / 3 + 1) return false;
+  if (a.mode == ROW_C2R ? (a.nin != 1 || a.nout != 0) : (a.nout != 1 || a.nin != 0)) return false;
+  switch (a.nx) {
+    case 64: row_plain_n<T, 64>(p, a, nrows); return true;
+    case 128: row_plain_n<T, 128>(p, a, nrows); return true;
+    case 256: row_plain_n<T, 256>(p, a, nrows); return true;
+    case 512: row_plain_n<T, 512>(p, a, nrows); return true;
+    default: return false;
+  }
+}
+
 template <typename T>
 int launch_row(ddx_plan* p, RowArgs<T>& a) {
   const int nseq = std::max((a.nin + 1) / 2, (a.nout + 1) / 2);
@@ -417,6 +441,10 @@ int launch_row(ddx_plan* p, RowArgs<T>& a) {
   else { cat = PC_ROW_R2C; bytes = nrows * (double)a.nout * (p->Kx * cb + a.nx * rb); }
   ProfScope prof(p, cat, bytes);
   if (a.mode == ROW_PHYS && !p->generic_only && launch_row_fast<T>(p, a, nrows)) {
+    LAUNCHCHK();
+    return 0;
+  }
+  if (a.mode != ROW_PHYS && !p->generic_only && launch_row_plain_fast<T>(p, a, nrows)) {
     LAUNCHCHK();
     return 0;
   }

@@ -449,6 +449,119 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
 }
 
 // ---------------------------------------------------------------------------
+// plain x-row transforms (ddx_bwd / ddx_fwd): two rows of one field per team
+// ---------------------------------------------------------------------------
+
+// c2r of rows 2t and 2t + 1: z = A + i B from the two half spectra, one
+// inverse transform, x rows = (Re z, Im z).  Loads as in row_phys_fast
+// (clamped addresses, zeroed afterwards, the k = 0 imaginary part dropped).
+template <typename T, int N>
+__global__ void __launch_bounds__(64) row_c2r_fast(const RowArgs<T> a, long nrows) {
+  using PL = RowPlan<N>;
+  constexpr int LANES = PL::LANES, E = PL::E, RPB = 64 / LANES;
+  constexpr int R0 = PL::R[0], RL = PL::R[PL::NP - 1];
+  constexpr int BUF = exch_size<N, PL>();
+  constexpr bool POW = sizeof(T) == 8;
+  __shared__ cpx<T> lds[RPB][BUF];
+  const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  const long ra = 2 * ((long)blockIdx.x * RPB + team), rb = ra + 1;
+  const bool live_a = ra < nrows, live_b = rb < nrows;
+  const long la_row = live_a ? ra : nrows - 1, lb_row = live_b ? rb : nrows - 1;
+  const int cx = a.kx - 1;
+  cpx<T> twc[PL::NP][E];
+  fill_twiddles<T, N, PL, false, POW>(twc, lane, a.tw);
+  const cpx<T>* __restrict__ ia = a.in[0] + la_row * a.kxp;
+  const cpx<T>* __restrict__ ib = a.in[0] + lb_row * a.kxp;
+  cpx<T> va[E], vb[E];
+#pragma unroll
+  for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < R0; ++rr) {
+      const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+      const int idx = m <= cx ? m : (m >= N - cx ? N - m : 0);
+      va[qq * R0 + rr] = ia[idx];
+      vb[qq * R0 + rr] = ib[idx];
+    }
+  cpx<T> v[E];
+#pragma unroll
+  for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < R0; ++rr) {
+      const int e = qq * R0 + rr;
+      const int m = in_pos<N, LANES, R0>(lane, qq, rr);
+      const bool keep = m <= cx || m >= N - cx;
+      const bool mirror = m >= N - cx;
+      const T z0 = m == 0 ? T(0) : T(1);
+      const T ay = (mirror ? -va[e].y : va[e].y) * z0;
+      const T by = live_b ? (mirror ? -vb[e].y : vb[e].y) * z0 : T(0);
+      const T bx = live_b ? vb[e].x : T(0);
+      v[e] = keep ? mk(va[e].x - by, ay + bx) : mk(T(0), T(0));
+    }
+  run_passes_cached<T, 1, N, PL, false, 1, POW>(v, lane, 0, lds[team], twc);
+  T* __restrict__ xa = a.xout[0] + ra * (long)N;
+#pragma unroll
+  for (int qq = 0; qq < E / RL; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < RL; ++rr) {
+      const int pos = final_pos<N, PL, false>(lane, qq, rr);
+      if (live_a) xa[pos] = v[qq * RL + rr].x;
+      if (live_b) xa[N + pos] = v[qq * RL + rr].y;
+    }
+}
+
+// r2c of rows 2t and 2t + 1: z = a + i b, one forward transform, half
+// spectra A = (Z_k + conj Z_{N-k}) / 2, B = (Z_k - conj Z_{N-k}) / 2i.  The
+// x loads sit at the first-pass positions of the reversed radix order.
+template <typename T, int N>
+__global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrows) {
+  using PL = RowPlan<N>;
+  constexpr int LANES = PL::LANES, E = PL::E, RPB = 64 / LANES;
+  constexpr int RL = PL::R[PL::NP - 1], RF = PL::R[0];
+  constexpr int BUF = exch_size<N, PL>();
+  constexpr bool POW = sizeof(T) == 8;
+  constexpr bool UNI = uniform_radix<PL>();
+  __shared__ cpx<T> lds[RPB][BUF];
+  const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  const long ra = 2 * ((long)blockIdx.x * RPB + team), rb = ra + 1;
+  const bool live_a = ra < nrows, live_b = rb < nrows;
+  const long la_row = live_a ? ra : nrows - 1, lb_row = live_b ? rb : nrows - 1;
+  cpx<T>* buf = lds[team];
+  cpx<T> twf[PL::NP][E];
+  fill_twiddles<T, N, PL, true, POW>(twf, lane, a.tw);
+  const T* __restrict__ xa = a.xin[0] + la_row * (long)N;
+  const T* __restrict__ xb = a.xin[0] + lb_row * (long)N;
+  cpx<T> v[E];
+#pragma unroll
+  for (int qq = 0; qq < E / RL; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < RL; ++rr) {
+      const int pos = final_pos<N, PL, false>(lane, qq, rr);
+      const T x0 = xa[pos], x1 = xb[pos];
+      v[qq * RL + rr] = mk(x0, live_b ? x1 : T(0));
+    }
+  run_passes_cached<T, -1, N, PL, true, 1, POW>(v, lane, 0, buf, twf);
+#pragma unroll
+  for (int qq = 0; qq < E / RF; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < RF; ++rr) buf[final_pos<N, PL, true>(lane, qq, rr)] = v[qq * RF + rr];
+  __syncthreads();
+  cpx<T>* __restrict__ oa = a.out[0] + ra * (long)a.kxp;
+  for (int k = lane; k < a.kxp; k += LANES) {
+    cpx<T> A = mk(T(0), T(0)), B = mk(T(0), T(0));
+    if (k < a.kx) {
+      const cpx<T> zk = buf[k];
+      const cpx<T> zm = conj(buf[(N - k) % N]);
+      const cpx<T> s = zk + zm, d = zk - zm;
+      A = mk(T(0.5) * s.x, T(0.5) * s.y);
+      B = mk(T(0.5) * d.y, T(-0.5) * d.x);
+    }
+    if (live_a) oa[k] = A;
+    if (live_b) oa[a.kxp + k] = B;
+  }
+  (void)UNI;
+}
+
+// ---------------------------------------------------------------------------
 // column kernel: W columns per workgroup, LANES threads per column
 // ---------------------------------------------------------------------------
 

@@ -12,7 +12,10 @@ from oracle import numpy_path as npth
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 8), (6, 8), (16, 16), (64, 64), (48, 64), (30, 20), (128, 128), (512, 512),
-          (8, 8, 8), (4, 6, 8), (16, 16, 16), (12, 10, 16), (48, 2, 48), (32, 32, 32)]
+          (8, 8, 8), (4, 6, 8), (16, 16, 16), (12, 10, 16), (48, 2, 48), (32, 32, 32),
+          # size-specialised x-row transforms (two rows per team; odd row counts
+          # leave the last team one row)
+          (5, 64), (3, 5, 64), (4, 6, 128), (12, 256), (2, 3, 512)]
 LENGTHS = {2: (1.0, 2.5), 3: (2 * np.pi, 1.5, 3.0)}
 FIELDS = {'hydro': ['u'], 'boussinesq': ['u', 'T'], 'mhd': ['u', 'B']}
 BDIR = {'x': 0, 'y': 1, 'z': 2}
