@@ -652,6 +652,52 @@ __global__ void pack_kernel(PackArgs a) {
   if (a.dst_f32) a.dst_f32[j] = v;
 }
 
+// One pixel per thread (Cp % 8 == 0, 16-byte aligned outputs): the C <= CMAX
+// source values are read once, unconditionally (channels past C read channel
+// 0 and are zeroed), and the Cp padded channels written as 16-byte chunks.
+template <bool F64, int CMAX>
+__global__ void pack_pixel_kernel(PackArgs a) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.npix) return;
+  float v[CMAX], x[CMAX];
+  const long o = p * a.pstride;
+#pragma unroll
+  for (int c = 0; c < CMAX; ++c) {
+    const int cc = c < a.C ? c : 0;
+    x[c] = F64 ? (float)((const double*)a.src[cc])[o] : ((const float*)a.src[cc])[o];
+  }
+  // empty asm statements taking the loaded values as operands, after all the
+  // loads: no load can be moved under the (uniform) channel test below, which
+  // would serialise them
+#pragma unroll
+  for (int c = 0; c < CMAX; ++c) asm volatile("" : "+v"(x[c]));
+#pragma unroll
+  for (int c = 0; c < CMAX; ++c) v[c] = c < a.C ? x[c] * a.scale[c < a.C ? c : 0] : 0.f;
+  for (int k = 0; k < a.Cp / 8; ++k) {
+    float w[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      w[e] = 0.f;
+#pragma unroll
+      for (int d = 0; d < CMAX; ++d)
+        if (d == 8 * k + e) w[e] = v[d];
+    }
+    if (a.dst_bf16) {
+      uint4 q;
+      q.x = (uint32_t)f2bf(w[0]) | ((uint32_t)f2bf(w[1]) << 16);
+      q.y = (uint32_t)f2bf(w[2]) | ((uint32_t)f2bf(w[3]) << 16);
+      q.z = (uint32_t)f2bf(w[4]) | ((uint32_t)f2bf(w[5]) << 16);
+      q.w = (uint32_t)f2bf(w[6]) | ((uint32_t)f2bf(w[7]) << 16);
+      *reinterpret_cast<uint4*>(&a.dst_bf16[p * a.Cp + 8 * k]) = q;
+    }
+    if (a.dst_f32) {
+      float4* d = reinterpret_cast<float4*>(&a.dst_f32[p * a.Cp + 8 * k]);
+      d[0] = make_float4(w[0], w[1], w[2], w[3]);
+      d[1] = make_float4(w[4], w[5], w[6], w[7]);
+    }
+  }
+}
+
 // Scatter channels: dst_c[p * pstride] = scale_c * src[p][c] (c < C), fp32 or fp64
 struct UnpackArgs {
   void* dst[DDX_CONV_MAXCH];
@@ -663,16 +709,22 @@ struct UnpackArgs {
 };
 
 __global__ void unpack_kernel(UnpackArgs a) {
-  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.npix * a.C) return;
-  const long p = j / a.C;
-  const int c = (int)(j - p * a.C);
-  const float v = a.src[p * a.Cp + c] * a.scale[c];
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.npix) return;
   const long o = p * a.pstride;
-  if (a.f64)
-    ((double*)a.dst[c])[o] = (double)v;
-  else
-    ((float*)a.dst[c])[o] = v;
+  const float* __restrict__ sp = a.src + p * a.Cp;
+  float v[DDX_CONV_MAXCH];
+#pragma unroll
+  for (int c = 0; c < DDX_CONV_MAXCH; ++c) v[c] = sp[c < a.C ? c : 0];   // loads first
+#pragma unroll
+  for (int c = 0; c < DDX_CONV_MAXCH; ++c) {
+    if (c >= a.C) break;
+    const float x = v[c] * a.scale[c];
+    if (a.f64)
+      ((double*)a.dst[c])[o] = (double)x;
+    else
+      ((float*)a.dst[c])[o] = x;
+  }
 }
 
 // Box filter: dst[Y][X] = mean of the f x f block of src starting at (f Y, f X)
@@ -887,7 +939,16 @@ int ddx_conv_pack(void* stream, int64_t npix, int C, int Cp, const void* const* 
   a.f64 = src_f64 ? 1 : 0;
   a.dst_bf16 = (uint16_t*)dst_bf16;
   a.dst_f32 = dst_f32;
-  hipLaunchKernelGGL(pack_kernel, dim3(nblk(npix * Cp, 256)), dim3(256), 0, (hipStream_t)stream, a);
+  const bool al = ((uintptr_t)dst_bf16 % 16) == 0 && ((uintptr_t)dst_f32 % 16) == 0;
+  const dim3 pg(nblk(npix, 256));
+  hipStream_t ps = (hipStream_t)stream;
+  if (Cp % 8 == 0 && al) {
+    if (a.f64 && C <= 4) hipLaunchKernelGGL((pack_pixel_kernel<true, 4>), pg, dim3(256), 0, ps, a);
+    else if (a.f64) hipLaunchKernelGGL((pack_pixel_kernel<true, DDX_CONV_MAXCH>), pg, dim3(256), 0, ps, a);
+    else if (C <= 4) hipLaunchKernelGGL((pack_pixel_kernel<false, 4>), pg, dim3(256), 0, ps, a);
+    else hipLaunchKernelGGL((pack_pixel_kernel<false, DDX_CONV_MAXCH>), pg, dim3(256), 0, ps, a);
+  } else
+    hipLaunchKernelGGL(pack_kernel, dim3(nblk(npix * Cp, 256)), dim3(256), 0, (hipStream_t)stream, a);
   CL_LAUNCHCHK("conv_pack");
   return 0;
 }
@@ -908,7 +969,7 @@ int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src,
   a.Cp = Cp;
   a.f64 = dst_f64 ? 1 : 0;
   a.src = src;
-  hipLaunchKernelGGL(unpack_kernel, dim3(nblk(npix * C, 256)), dim3(256), 0, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(unpack_kernel, dim3(nblk(npix, 256)), dim3(256), 0, (hipStream_t)stream, a);
   CL_LAUNCHCHK("conv_unpack");
   return 0;
 }
