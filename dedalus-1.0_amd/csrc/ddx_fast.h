@@ -568,15 +568,16 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
 // First-pass inputs of one column: the stored rows (compact layouts leave out
 // the rows the 2/3 rule removes), or with SPECTRAL the recipe's NT terms
 // coef * (i k)^[dim >= 0] * src evaluated on the fly.  Every load is issued
-// unconditionally, from a clamped in-range address (column 0 past the last
-// column, row 0 for rows left out), and unused values are zeroed by a select
+// unconditionally, from a clamped in-range address (the last column for the
+// lanes past it, which lies in the same tile and lines; row 0 for rows left
+// out), and unused values are zeroed by a select
 // afterwards: loads under lane-dependent or per-term branches each end in a
 // wait for memory, which serialises the column's loads.
 template <typename T, int N, class PL, bool SPECTRAL, int NT>
 __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>& a, int lane, int f,
                                            long slab, int col, bool live) {
   constexpr int LANES = PL::LANES, E = PL::E, R0 = PL::R[0];
-  const int colc = live ? col : 0;
+  const int colc = live ? col : a.ncols - 1;
   // per term: source, coefficient and wavenumber table (the kx table, valid
   // in every pass, stands in for terms without a derivative)
   constexpr int NTA = NT > 0 ? NT : 1;
