@@ -48,7 +48,7 @@ def compare(data, ref, tol=1e-12):
 
 
 def test_taylor_green_64_closed_form():
-    """2D TG 64^2, nu = 1e-3, RK2mid dt = 1e-3, 200 steps."""
+    """The 2D Taylor-Green workload: 64^2, nu = 1e-3, RK2mid dt = 1e-3, 200 steps."""
     m = mods()
     from dedalus.init_cond.seeded import workload
     case("taylor-green 64^2 200 steps")
