@@ -409,7 +409,8 @@ template <typename T, int N>
 void row_plain_n(ddx_plan* p, const RowArgs<T>& a, long nrows) {
   constexpr int RPB = 64 / RowPlan<N>::LANES;
   const long teams = (nrows + 1) / 2;
-  const dim3 grid((unsigned)((teams + RPB - 1) / RPB));
+  const int nf = a.mode == ROW_C2R ? a.nin : a.nout;   // grid.y: the field
+  const dim3 grid((unsigned)((teams + RPB - 1) / RPB), (unsigned)nf);
   if (a.mode == ROW_C2R) hipLaunchKernelGGL((row_c2r_fast<T, N>), grid, dim3(64), 0, p->stream, a, nrows);
   else hipLaunchKernelGGL((row_r2c_fast<T, N>), grid, dim3(64), 0, p->stream, a, nrows);
 }
@@ -417,7 +418,7 @@ void row_plain_n(ddx_plan* p, const RowArgs<T>& a, long nrows) {
 template <typename T>
 bool launch_row_plain_fast(ddx_plan* p, const RowArgs<T>& a, long nrows) {
   if (a.kx > a.nx / 3 + 1) return false;
-  if (a.mode == ROW_C2R ? (a.nin != 1 || a.nout != 0) : (a.nout != 1 || a.nin != 0)) return false;
+  if (a.mode == ROW_C2R ? (a.nin < 1 || a.nout != 0) : (a.nout < 1 || a.nin != 0)) return false;
   switch (a.nx) {
     case 64: row_plain_n<T, 64>(p, a, nrows); return true;
     case 128: row_plain_n<T, 128>(p, a, nrows); return true;
@@ -431,6 +432,18 @@ template <typename T>
 int launch_row(ddx_plan* p, RowArgs<T>& a) {
   const int nseq = std::max((a.nin + 1) / 2, (a.nout + 1) / 2);
   const size_t lds = (size_t)2 * nseq * a.nx * sizeof(cpx<T>);
+  if (lds > 65536 && a.mode != ROW_PHYS && std::max(a.nin, a.nout) > 1) {
+    // several plain fields over the LDS of the generic kernel: one at a time
+    const int nf = std::max(a.nin, a.nout);
+    for (int f = 0; f < nf; ++f) {
+      RowArgs<T> b = a;
+      if (a.mode == ROW_C2R) { b.nin = 1; b.in[0] = a.in[f]; b.xout[0] = a.xout[f]; }
+      else { b.nout = 1; b.xin[0] = a.xin[f]; b.out[0] = a.out[f]; }
+      int rc = launch_row(p, b);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   if (lds > 65536) return fail(DDX_ERR_UNSUPPORTED, "x rows too long for LDS");
   const long nrows = (long)p->ay.n * p->az.n;
   const double cb = sizeof(cpx<T>), rb = sizeof(T);
@@ -468,71 +481,66 @@ void row_defaults(const ddx_plan* p, RowArgs<T>& a) {
 // transforms
 // ---------------------------------------------------------------------------
 
+// nf fields at once: every pass is one launch over all the fields
 template <typename T>
-int fwd_impl(ddx_plan* p, const T* x, cpx<T>* kc) {
-  int rc = ensure_ws(p, 1);
+int fwd_impl(ddx_plan* p, int nf, const T* const* x, cpx<T>* const* kc) {
+  int rc = ensure_ws(p, nf);
   if (rc) return rc;
+  const long fs = ws_field_stride(p);
   cpx<T>* A = (cpx<T>*)p->wsA;
   cpx<T>* B = (cpx<T>*)p->wsB;
   RowArgs<T> r;
   row_defaults(p, r);
   r.mode = ROW_R2C;
-  r.nin = 0; r.nout = 1;
-  r.xin[0] = x;
-  r.out[0] = A;
+  r.nin = 0; r.nout = nf;
+  for (int f = 0; f < nf; ++f) { r.xin[f] = x[f]; r.out[f] = A + f * fs; }
   if ((rc = launch_row(p, r))) return rc;
   ColArgs<T> c;
   col_defaults(c);
   geom_fy(p, c);
-  c.in[0] = A;
+  for (int f = 0; f < nf; ++f) c.in[f] = A + f * fs;
   if (!p->zfft) {
-    c.out[0] = kc;
+    for (int f = 0; f < nf; ++f) c.out[f] = kc[f];
     c.scale = T(1.0 / (double)p->nper);
-    return launch_col(p, c, p->az.n, 1, -1, false, PC_COL_FWD_Y);
+    return launch_col(p, c, p->az.n, nf, -1, false, PC_COL_FWD_Y);
   }
-  c.out[0] = B;
-  if ((rc = launch_col(p, c, p->az.n, 1, -1, false, PC_COL_FWD_Y))) return rc;
+  for (int f = 0; f < nf; ++f) c.out[f] = B + f * fs;
+  if ((rc = launch_col(p, c, p->az.n, nf, -1, false, PC_COL_FWD_Y))) return rc;
   col_defaults(c);
   geom_fz(p, c);
-  c.in[0] = B;
-  c.out[0] = kc;
+  for (int f = 0; f < nf; ++f) { c.in[f] = B + f * fs; c.out[f] = kc[f]; }
   c.scale = T(1.0 / (double)p->nper);
-  return launch_col(p, c, p->Ky, 1, -1, false, PC_COL_FWD_Z);
+  return launch_col(p, c, p->Ky, nf, -1, false, PC_COL_FWD_Z);
 }
 
 template <typename T>
-int bwd_impl(ddx_plan* p, const cpx<T>* kc, T* x) {
-  int rc = ensure_ws(p, 1);
+int bwd_impl(ddx_plan* p, int nf, const cpx<T>* const* kc, T* const* x) {
+  int rc = ensure_ws(p, nf);
   if (rc) return rc;
+  const long fs = ws_field_stride(p);
   cpx<T>* A = (cpx<T>*)p->wsA;
   cpx<T>* B = (cpx<T>*)p->wsB;
   ColArgs<T> c;
-  const cpx<T>* ymix;
   if (p->zfft) {
     col_defaults(c);
     geom_bz(p, c);
-    c.in[0] = kc;
-    c.out[0] = A;
-    if ((rc = launch_col(p, c, p->Ky, 1, 1, false, PC_COL_BWD_Z))) return rc;
+    for (int f = 0; f < nf; ++f) { c.in[f] = kc[f]; c.out[f] = A + f * fs; }
+    if ((rc = launch_col(p, c, p->Ky, nf, 1, false, PC_COL_BWD_Z))) return rc;
     col_defaults(c);
     geom_by(p, c);
-    c.in[0] = A;
-    c.out[0] = B;
-    if ((rc = launch_col(p, c, p->az.n, 1, 1, false, PC_COL_BWD_Y))) return rc;
+    for (int f = 0; f < nf; ++f) { c.in[f] = A + f * fs; c.out[f] = B + f * fs; }
+    if ((rc = launch_col(p, c, p->az.n, nf, 1, false, PC_COL_BWD_Y))) return rc;
   } else {
     col_defaults(c);
     geom_by(p, c);
-    c.in[0] = kc;
-    c.out[0] = B;
-    if ((rc = launch_col(p, c, p->az.n, 1, 1, false, PC_COL_BWD_Y))) return rc;
+    for (int f = 0; f < nf; ++f) { c.in[f] = kc[f]; c.out[f] = B + f * fs; }
+    if ((rc = launch_col(p, c, p->az.n, nf, 1, false, PC_COL_BWD_Y))) return rc;
   }
-  ymix = B;
   RowArgs<T> r;
   row_defaults(p, r);
   r.mode = ROW_C2R;
-  r.nin = 1; r.nout = 0;
-  r.in[0] = ymix;
-  r.xout[0] = x;
+  r.nin = nf; r.nout = 0;
+  for (int f = 0; f < nf; ++f) { r.in[f] = B + f * fs; r.xout[f] = x[f]; }
   return launch_row(p, r);
 }
 
@@ -1211,14 +1219,32 @@ int ddx_unpack(ddx_plan* p, const void* kc, void* kref) {
 
 int ddx_fwd(ddx_plan* p, const void* x, void* kc) {
   if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
-  if (p->precision == DDX_F64) return fwd_impl<double>(p, (const double*)x, (cpx<double>*)kc);
-  return fwd_impl<float>(p, (const float*)x, (cpx<float>*)kc);
+  return ddx_fwd_many(p, 1, &x, &kc);
 }
 
 int ddx_bwd(ddx_plan* p, const void* kc, void* x) {
   if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
-  if (p->precision == DDX_F64) return bwd_impl<double>(p, (const cpx<double>*)kc, (double*)x);
-  return bwd_impl<float>(p, (const cpx<float>*)kc, (float*)x);
+  return ddx_bwd_many(p, 1, &kc, &x);
+}
+
+int ddx_fwd_many(ddx_plan* p, int nf, const void* const* x, void* const* kc) {
+  if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
+  if (nf < 1 || nf > DDX_MAXF) return fail(DDX_ERR_ARG, "1..16 fields");
+  for (int f = 0; f < nf; ++f)
+    if (!x[f] || !kc[f]) return fail(DDX_ERR_ARG, "null field pointer");
+  if (p->precision == DDX_F64)
+    return fwd_impl<double>(p, nf, (const double* const*)x, (cpx<double>* const*)kc);
+  return fwd_impl<float>(p, nf, (const float* const*)x, (cpx<float>* const*)kc);
+}
+
+int ddx_bwd_many(ddx_plan* p, int nf, const void* const* kc, void* const* x) {
+  if (check_plan(p) || !x || !kc) return fail(DDX_ERR_ARG, "null argument");
+  if (nf < 1 || nf > DDX_MAXF) return fail(DDX_ERR_ARG, "1..16 fields");
+  for (int f = 0; f < nf; ++f)
+    if (!x[f] || !kc[f]) return fail(DDX_ERR_ARG, "null field pointer");
+  if (p->precision == DDX_F64)
+    return bwd_impl<double>(p, nf, (const cpx<double>* const*)kc, (double* const*)x);
+  return bwd_impl<float>(p, nf, (const cpx<float>* const*)kc, (float* const*)x);
 }
 
 int ddx_deriv(ddx_plan* p, const void* kc, void* out, int dim) {

@@ -449,7 +449,8 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
 }
 
 // ---------------------------------------------------------------------------
-// plain x-row transforms (ddx_bwd / ddx_fwd): two rows of one field per team
+// plain x-row transforms (ddx_bwd / ddx_fwd): two rows of one field per team,
+// the field (of several in one call) in blockIdx.y
 // ---------------------------------------------------------------------------
 
 // c2r of rows 2t and 2t + 1: z = A + i B from the two half spectra, one
@@ -470,8 +471,9 @@ __global__ void __launch_bounds__(64) row_c2r_fast(const RowArgs<T> a, long nrow
   const int cx = a.kx - 1;
   cpx<T> twc[PL::NP][E];
   fill_twiddles<T, N, PL, false, POW>(twc, lane, a.tw);
-  const cpx<T>* __restrict__ ia = a.in[0] + la_row * a.kxp;
-  const cpx<T>* __restrict__ ib = a.in[0] + lb_row * a.kxp;
+  const int f = blockIdx.y;   // the field
+  const cpx<T>* __restrict__ ia = a.in[f] + la_row * a.kxp;
+  const cpx<T>* __restrict__ ib = a.in[f] + lb_row * a.kxp;
   cpx<T> va[E], vb[E];
 #pragma unroll
   for (int qq = 0; qq < E / R0; ++qq)
@@ -498,7 +500,7 @@ __global__ void __launch_bounds__(64) row_c2r_fast(const RowArgs<T> a, long nrow
       v[e] = keep ? mk(va[e].x - by, ay + bx) : mk(T(0), T(0));
     }
   run_passes_cached<T, 1, N, PL, false, 1, POW>(v, lane, 0, lds[team], twc);
-  T* __restrict__ xa = a.xout[0] + ra * (long)N;
+  T* __restrict__ xa = a.xout[f] + ra * (long)N;
 #pragma unroll
   for (int qq = 0; qq < E / RL; ++qq)
 #pragma unroll
@@ -528,8 +530,9 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
   cpx<T>* buf = lds[team];
   cpx<T> twf[PL::NP][E];
   fill_twiddles<T, N, PL, true, POW>(twf, lane, a.tw);
-  const T* __restrict__ xa = a.xin[0] + la_row * (long)N;
-  const T* __restrict__ xb = a.xin[0] + lb_row * (long)N;
+  const int f = blockIdx.y;   // the field
+  const T* __restrict__ xa = a.xin[f] + la_row * (long)N;
+  const T* __restrict__ xb = a.xin[f] + lb_row * (long)N;
   cpx<T> v[E];
 #pragma unroll
   for (int qq = 0; qq < E / RL; ++qq)
@@ -545,7 +548,7 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
 #pragma unroll
     for (int rr = 0; rr < RF; ++rr) buf[final_pos<N, PL, true>(lane, qq, rr)] = v[qq * RF + rr];
   __syncthreads();
-  cpx<T>* __restrict__ oa = a.out[0] + ra * (long)a.kxp;
+  cpx<T>* __restrict__ oa = a.out[f] + ra * (long)a.kxp;
   for (int k = lane; k < a.kxp; k += LANES) {
     cpx<T> A = mk(T(0), T(0)), B = mk(T(0), T(0));
     if (k < a.kx) {

@@ -82,6 +82,8 @@ SIGNATURES = {
     "ddx_unpack": (_I, [_VP, _VP, _VP]),
     "ddx_fwd": (_I, [_VP, _VP, _VP]),
     "ddx_bwd": (_I, [_VP, _VP, _VP]),
+    "ddx_fwd_many": (_I, [_VP, _I, _PP, _PP]),
+    "ddx_bwd_many": (_I, [_VP, _I, _PP, _PP]),
     "ddx_deriv": (_I, [_VP, _VP, _VP, _I]),
     "ddx_project": (_I, [_VP, _PP]),
     "ddx_update": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _D, _I, _D]),
@@ -351,6 +353,20 @@ class Plan:
 
     def bwd(self, kc, x):
         _check(_lib.ddx_bwd(self.handle, int(kc), int(x)))
+
+    def fwd_many(self, xs, kcs):
+        """fwd of several fields in one call (one launch per pass)."""
+        n = len(xs)
+        a = (ctypes.c_void_p * n)(*[int(b) for b in xs])
+        b = (ctypes.c_void_p * n)(*[int(b) for b in kcs])
+        _check(_lib.ddx_fwd_many(self.handle, n, a, b))
+
+    def bwd_many(self, kcs, xs):
+        """bwd of several fields in one call (one launch per pass)."""
+        n = len(kcs)
+        a = (ctypes.c_void_p * n)(*[int(b) for b in kcs])
+        b = (ctypes.c_void_p * n)(*[int(b) for b in xs])
+        _check(_lib.ddx_bwd_many(self.handle, n, a, b))
 
     def deriv(self, kc, out, dim):
         _check(_lib.ddx_deriv(self.handle, int(kc), int(out), int(dim)))

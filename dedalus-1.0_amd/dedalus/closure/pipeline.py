@@ -308,8 +308,8 @@ class CoarseRun:
                  scale=self.in_scale, bf16=self.inf.x, f32=False)
             out = self.inf.run()
             unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale)
+            C.fwd_many(self.fx, self.fk)
             for c in range(2):
-                C.fwd(self.fx[c], self.fk[c])
                 C.kcombine(self.fk[c], [(self.forcing[c], 1.0, -1)], accumulate=True)
             forcing = self.fk
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], self.bufs[2],

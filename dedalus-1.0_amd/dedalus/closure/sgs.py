@@ -108,11 +108,10 @@ def vorticity_k(plan, u, out):
 
 
 def features_x(plan, u, xbufs, work_k):
-    """x-space u, v, omega of a 2D velocity into three x buffers of the plan."""
-    plan.bwd(u[0], xbufs[0])
-    plan.bwd(u[1], xbufs[1])
+    """x-space u, v, omega of a 2D velocity into three x buffers of the plan
+    (the three backward transforms as one call)."""
     vorticity_k(plan, u, work_k)
-    plan.bwd(work_k, xbufs[2])
+    plan.bwd_many([u[0], u[1], work_k], xbufs[:3])
 
 
 def shell_spectrum(kref_comps, shape, length):

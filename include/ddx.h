@@ -145,6 +145,16 @@ int ddx_fwd(ddx_plan* plan, const void* x, void* kc);
  * the kx = 0 plane contributes its Hermitian part, as irfftn does. */
 int ddx_bwd(ddx_plan* plan, const void* kc, void* x);
 
+/* ddx_fwd / ddx_bwd of nf <= 16 fields in one call: each pass is one launch
+ * over all the fields.  The results equal nf single calls bit for bit for
+ * x extents 64..512 (size-specialised rows); other extents pair two fields
+ * per complex row transform, equal to rounding.  The
+ * reference transforms one component per call (representations.py:335-357);
+ * this batches the components a caller needs together, for small grids where
+ * launches, not bytes, set the time. */
+int ddx_fwd_many(ddx_plan* plan, int nf, const void* const* x, void* const* kc);
+int ddx_bwd_many(ddx_plan* plan, int nf, const void* const* kc, void* const* x);
+
 /* ---- per-operation kernels ------------------------------------------------- */
 
 /* out = i k_dim * kc, dim 0 = x, 1 = y, 2 = z (representations.py:419-425). */
