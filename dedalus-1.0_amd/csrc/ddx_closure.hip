@@ -701,6 +701,7 @@ __global__ void pack_pixel_kernel(PackArgs a) {
 // Scatter channels: dst_c[p * pstride] = scale_c * src[p][c] (c < C), fp32 or fp64
 struct UnpackArgs {
   void* dst[DDX_CONV_MAXCH];
+  const void* add[DDX_CONV_MAXCH];   // all null, or one per channel
   float scale[DDX_CONV_MAXCH];
   long pstride;
   long npix;
@@ -721,9 +722,9 @@ __global__ void unpack_kernel(UnpackArgs a) {
     if (c >= a.C) break;
     const float x = v[c] * a.scale[c];
     if (a.f64)
-      ((double*)a.dst[c])[o] = (double)x;
+      ((double*)a.dst[c])[o] = (double)x + (a.add[0] ? ((const double*)a.add[c])[o] : 0.0);
     else
-      ((float*)a.dst[c])[o] = x;
+      ((float*)a.dst[c])[o] = x + (a.add[0] ? ((const float*)a.add[c])[o] : 0.f);
   }
 }
 
@@ -954,13 +955,15 @@ int ddx_conv_pack(void* stream, int64_t npix, int C, int Cp, const void* const* 
 }
 
 int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src, void* const* dst,
-                    int64_t pstride, int dst_f64, const float* scale) {
+                    int64_t pstride, int dst_f64, const float* scale, const void* const* add) {
   if (!src || !dst || npix < 1 || C < 1 || C > DDX_CONV_MAXCH || Cp < C || pstride < 1)
     return ddx::set_error(DDX_ERR_ARG, "bad unpack arguments");
   UnpackArgs a{};
   for (int c = 0; c < C; ++c) {
     if (!dst[c]) return ddx::set_error(DDX_ERR_ARG, "null channel destination");
+    if (add && !add[c]) return ddx::set_error(DDX_ERR_ARG, "null channel addend");
     a.dst[c] = dst[c];
+    a.add[c] = add ? add[c] : nullptr;
     a.scale[c] = scale ? scale[c] : 1.0f;
   }
   a.pstride = pstride;

@@ -103,7 +103,7 @@ SIGNATURES = {
     "ddx_conv_wprep": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _VP]),
     "ddx_conv_wunpack": (_I, [_VP, _VP, _I, _I, _I, _I, _VP]),
     "ddx_conv_pack": (_I, [_VP, ctypes.c_int64, _I, _I, _PP, ctypes.c_int64, _I, _VP, _VP, _VP]),
-    "ddx_conv_unpack": (_I, [_VP, ctypes.c_int64, _I, _I, _VP, _PP, ctypes.c_int64, _I, _VP]),
+    "ddx_conv_unpack": (_I, [_VP, ctypes.c_int64, _I, _I, _VP, _PP, ctypes.c_int64, _I, _VP, _PP]),
     "ddx_box_filter": (_I, [_VP, _I, _I, _I, _I, _VP, _VP]),
     "ddx_mfma_probe": (_I, [_VP, _I, _VP, _VP, _VP]),
 }

@@ -157,16 +157,18 @@ def pack(srcs, npix, cp, pstride=1, scale=None, bf16=True, f32=False):
     return ob, of
 
 
-def unpack(src, C, dsts, f64, pstride=1, scale=None):
+def unpack(src, C, dsts, f64, pstride=1, scale=None, add=None):
     """Scatter the first C channels of src [npix, Cp] float32 to channel
-    destinations (device pointers, float64 if f64)."""
+    destinations (device pointers, float64 if f64), optionally adding the
+    arrays ``add`` (device pointers of the destination type)."""
     lib = _lib()
     sc = None
     if scale is not None:
         sc = (ctypes.c_float * C)(*[float(v) for v in scale])
+    ad = _hip.ptr_array([int(d) for d in add]) if add is not None else None
     _check(lib.ddx_conv_unpack(_stream(), src.shape[0], C, src.shape[1], _ptr(src),
                                _hip.ptr_array([int(d) for d in dsts]), int(pstride),
-                               1 if f64 else 0, sc))
+                               1 if f64 else 0, sc, ad))
 
 
 class _NetFn(torch.autograd.Function):

@@ -287,6 +287,11 @@ class CoarseRun:
             self.fx = [C.alloc_x() for _ in range(2)]
             self.fk = [C.alloc_k() for _ in range(2)]
             self.wk = C.alloc_k()
+            # the coarse forcing in x space joins the network tendency at the
+            # unpack, so one forward transform gives their sum
+            self.fcx = [C.alloc_x() for _ in range(2)]
+            with torch.cuda.stream(pipe.stream):
+                C.bwd_many(self.forcing, self.fcx)
 
     def set_state(self, ubar):
         """Copy a coarse velocity (two k buffers of the run's plan, or of the
@@ -307,10 +312,9 @@ class CoarseRun:
             pack([(int(b), True) for b in self.xb], self.npix, self.inf.x.shape[1],
                  scale=self.in_scale, bf16=self.inf.x, f32=False)
             out = self.inf.run()
-            unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale)
+            unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale,
+                   add=[int(b) for b in self.fcx])
             C.fwd_many(self.fx, self.fk)
-            for c in range(2):
-                C.kcombine(self.fk[c], [(self.forcing[c], 1.0, -1)], accumulate=True)
             forcing = self.fk
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], self.bufs[2],
                1, self.dt, forcing=forcing)

@@ -81,10 +81,13 @@ int ddx_conv_pack(void* stream, int64_t npix, int C, int Cp, const void* const* 
                   int64_t pstride, int src_f64, const float* scale, void* dst_bf16,
                   float* dst_f32);
 
-/* Scatter: dst[c][p * pstride] = scale[c] * src[p][c] for c < C (src has Cp
- * channels per pixel); dst entries float64 (dst_f64 = 1) or float32. */
+/* Scatter: dst[c][p * pstride] = scale[c] * src[p][c] (+ add[c][p * pstride])
+ * for c < C (src has Cp channels per pixel); dst and add entries float64
+ * (dst_f64 = 1) or float32; add may be NULL.  With add, a constant x-space
+ * forcing joins the network's tendency before its forward transform. */
 int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src,
-                    void* const* dst, int64_t pstride, int dst_f64, const float* scale);
+                    void* const* dst, int64_t pstride, int dst_f64, const float* scale,
+                    const void* const* add);
 
 /* Coarse-graining: dst[Y][X] = mean of src over the factor x factor block at
  * (factor*Y, factor*X); src is ny x nx (float64 if f64, else float32). */

@@ -182,6 +182,21 @@ def test_colsum_pack_unpack():
     torch.cuda.synchronize()
     for c in range(3):
         assert torch.equal(outs[c].cpu(), srcs[c].cpu().reshape(-1).float().double())
+    # unpack with addends: dst = scale * src + add, float64 and float32
+    adds = [torch.randn(32 * 48, dtype=torch.float64, generator=g).to(d) for _ in range(3)]
+    net.unpack(of, 3, [o.data_ptr() for o in outs], f64=True, scale=[2.0, 0.5, -1.0],
+               add=[x.data_ptr() for x in adds])
+    outs32 = [torch.empty(32 * 48, dtype=torch.float32, device=d) for _ in range(2)]
+    adds32 = [x.float() for x in adds[:2]]
+    net.unpack(of, 2, [o.data_ptr() for o in outs32], f64=False, scale=[2.0, 0.5],
+               add=[x.data_ptr() for x in adds32])
+    torch.cuda.synchronize()
+    for c in range(3):
+        expect = (of[:, c].cpu() * [2.0, 0.5, -1.0][c]).double() + adds[c].cpu()
+        assert torch.equal(outs[c].cpu(), expect)
+    for c in range(2):
+        expect = of[:, c].cpu() * [2.0, 0.5][c] + adds32[c].cpu()
+        assert torch.equal(outs32[c].cpu(), expect)
 
 
 @pytest.mark.parametrize("f64", [True, False])
