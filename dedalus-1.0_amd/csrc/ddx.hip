@@ -128,6 +128,9 @@ struct ddx_plan {
   // ddx_step applies each ETD sweep inside the preceding RHS epilogue
   // (DDX_NO_FUSED_UPDATE=1 runs them as separate launches)
   bool fuse_update = true;
+  // single 2D grids with 64-point y axes run the forward y pass and the
+  // k-space epilogue as one kernel (DDX_NO_FUSED_YEPI=1 runs them as two)
+  bool fuse_yepi = true;
   int nbatch = 1;
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
@@ -147,12 +150,12 @@ size_t real_bytes(const ddx_plan* p) { return p->precision == DDX_F64 ? 8 : 4; }
 enum ProfCat {
   PC_COL_BWD_Z_SPECTRAL, PC_COL_BWD_Y_SPECTRAL, PC_COL_BWD_Z, PC_COL_BWD_Y,
   PC_COL_FWD_Y, PC_COL_FWD_Z, PC_ROW_PHYS, PC_ROW_C2R, PC_ROW_R2C,
-  PC_EPILOGUE, PC_UPDATE, PC_LAYOUT, PC_POINTWISE, PC_COPY, PC_NCAT
+  PC_EPILOGUE, PC_UPDATE, PC_LAYOUT, PC_POINTWISE, PC_COPY, PC_YEPI, PC_NCAT
 };
 const char* const kCatNames[PC_NCAT] = {
   "col_bwd_z_spectral", "col_bwd_y_spectral", "col_bwd_z", "col_bwd_y",
   "col_fwd_y", "col_fwd_z", "row_phys", "row_c2r", "row_r2c",
-  "epilogue", "update", "layout", "pointwise", "copy"};
+  "epilogue", "update", "layout", "pointwise", "copy", "col_fwd_y_epilogue"};
 
 hipEvent_t take_event(ddx_plan* p) {
   if (!p->evfree.empty()) {
@@ -671,7 +674,11 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int o = 0; o < L.np; ++o) r.out[o] = A + o * fs;
   if ((rc = launch_row(p, r))) return rc;
 
-  // forward column passes
+  // forward column passes (single 2D grids with 64-point y axes: the y pass
+  // runs inside the epilogue; batched plans keep the two launches, which
+  // measured as fast for 256 members)
+  const bool yepi = !p->zfft && nd == 2 && p->nbatch == 1 && p->fuse_yepi && !p->generic_only &&
+                    p->ay.n == 64 && p->Ky <= 2 * ((64 + 2) / 3) + 1 && L.np <= 3;
   cpx<T>* Uc;
   col_defaults(c);
   geom_fy(p, c);
@@ -684,6 +691,8 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
     for (int o = 0; o < L.np; ++o) { c.in[o] = B + o * fs; c.out[o] = A + o * fs; }
     if ((rc = launch_col(p, c, p->Ky, L.np, -1, false, PC_COL_FWD_Z))) return rc;
     Uc = A;
+  } else if (yepi) {
+    Uc = A;   // the x-row products, (members, 64, Kxp) per field
   } else {
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
     if ((rc = launch_col(p, c, p->az.n, L.np, -1, false, PC_COL_FWD_Y))) return rc;
@@ -728,6 +737,18 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int s = 0; s < L.nstate; ++s) nforce += e.forcing[s] ? 1 : 0;
   const double kvalid = (double)p->Ky * p->Kz * p->Kx;
   const int extra = prm->physics == DDX_BOUSSINESQ ? 2 : 0;
+  if (yepi) {
+    // the y pass's reads of the x-row products replace the epilogue's reads
+    // of its output
+    const double yin = (double)p->az.n * p->ay.n * p->Kx * L.np;
+    ProfScope prof(p, PC_YEPI,
+                   (yin + kvalid * (L.nstate + extra + nforce + upd_arrays * L.nstate)) * sizeof(cpx<T>));
+    e.tw_y = (const cpx<T>*)p->tw.at(p->ay.n);
+    const dim3 grid((p->Kxp + YEPI_W - 1) / YEPI_W, p->az.n);
+    hipLaunchKernelGGL((yepi64_kernel<T>), grid, dim3(64 * YEPI_WAVES), 0, p->stream, e);
+    LAUNCHCHK();
+    return 0;
+  }
   ProfScope prof(p, PC_EPILOGUE,
                  kvalid * (L.np + L.nstate + extra + nforce + upd_arrays * L.nstate) * sizeof(cpx<T>));
   hipLaunchKernelGGL((epilogue_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
@@ -955,6 +976,8 @@ int plan_create(int ndim, const int64_t* xshape, const double* length, int preci
     p->use_graph = !(ng && ng[0] == '1');
     const char* nf = getenv("DDX_NO_FUSED_UPDATE");
     p->fuse_update = !(nf && nf[0] == '1');
+    const char* ny = getenv("DDX_NO_FUSED_YEPI");
+    p->fuse_yepi = !(ny && ny[0] == '1');
   }
   p->ndim = ndim;
   p->precision = precision;

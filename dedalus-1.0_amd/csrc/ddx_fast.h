@@ -709,4 +709,85 @@ col_fft_fast(const ColArgs<T> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// forward y pass fused with the k-space epilogue (2D, n_y = 64)
+// ---------------------------------------------------------------------------
+
+constexpr int YEPI_W = 8;      // kx columns per workgroup
+constexpr int YEPI_WAVES = 4;  // wavefront f transforms product field f
+
+// One workgroup: YEPI_W kx columns of one member, every product field (2 or
+// 3).  Wavefront f runs the y transform of field f (the column plan of 64:
+// 8 lanes x 8 columns), so the fields are transformed side by side; a
+// wavefront without a field repeats field 0 and discards it, which keeps the
+// workgroup barriers uniform.  The kept ky modes go to LDS, and then all
+// wavefronts run the epilogue (epi_mode) one mode per thread.  This saves
+// the compact products' round trip through HBM and, on the 64^2 grids where
+// launch latency sets the time, one launch per right-hand side.
+// a.U[f]: the x-row products, layout (members, 64, Kxp).
+template <typename T>
+__global__ void __launch_bounds__(64 * YEPI_WAVES) yepi64_kernel(const EpiArgs<T> a) {
+  constexpr int N = 64;
+  using PL = ColPlan<N>;
+  constexpr int LANES = PL::LANES, E = PL::E, W = YEPI_W;
+  static_assert(LANES * W == 64, "one wavefront per field");
+  constexpr int R0 = PL::R[0], RL = PL::R[PL::NP - 1];
+  constexpr int BUF = exch_size<N, PL>();
+  constexpr int KYMAX = 2 * ((N + 2) / 3) + 1;   // bound on the kept rows
+  constexpr bool POW = sizeof(T) == 8;
+  __shared__ cpx<T> lds[YEPI_WAVES][BUF * W];
+  __shared__ cpx<T> stage[3][KYMAX * W];
+  const KGeom& g = a.g;
+  const int wave = threadIdx.x / 64, t = threadIdx.x % 64;
+  const int c = t % W, lane = t / W;
+  const int col = blockIdx.x * W + c;
+  const long slab = blockIdx.y;
+  const long base = slab * (long)N * g.Kxp + (col < g.Kxp ? col : g.Kxp - 1);
+  const int cy = (g.Ky - 1) / 2;
+  const bool bouss = a.physics == PH_BOUSS, mhd = a.physics == PH_MHD;
+  const int nf = 2 + ((bouss || mhd) ? 1 : 0);
+  const int nvar = 2 + (bouss ? 1 : (mhd ? 2 : 0));
+  const int f = wave < nf ? wave : 0;
+  cpx<T> twc[PL::NP][E];
+  fill_twiddles<T, N, PL, false, POW>(twc, lane, a.tw_y);
+  cpx<T> v[E];
+  const cpx<T>* __restrict__ src = (f == 0 ? a.U[0] : (f == 1 ? a.U[1] : a.U[2]));
+#pragma unroll
+  for (int qq = 0; qq < E / R0; ++qq)
+#pragma unroll
+    for (int rr = 0; rr < R0; ++rr)
+      v[qq * R0 + rr] = src[base + (long)in_pos<N, LANES, R0>(lane, qq, rr) * g.Kxp];
+  run_passes_cached<T, -1, N, PL, false, W, POW>(v, lane, c, lds[wave], twc);
+  if (wave < nf) {
+#pragma unroll
+    for (int qq = 0; qq < E / RL; ++qq)
+#pragma unroll
+      for (int rr = 0; rr < RL; ++rr) {
+        const int m = final_pos<N, PL, false>(lane, qq, rr);
+        const int j = m <= cy ? m : (m >= N - cy ? m - (N - g.Ky) : -1);
+        if (j >= 0) stage[wave][j * W + c] = v[qq * RL + rr];
+      }
+  }
+  __syncthreads();
+  const cpx<T> z = mk(T(0), T(0));
+  for (int i = threadIdx.x; i < g.Ky * W; i += blockDim.x) {
+    const int j = i / W, cc = i - j * W;
+    const int cl = blockIdx.x * W + cc;
+    if (cl >= g.Kxp) continue;
+    const long idx = ((long)j * g.Kz + slab) * g.Kxp + cl;
+    if (cl >= g.Kx) {
+      epi_pad(a, idx, nvar);
+      continue;
+    }
+    cpx<T> tval = z, bval = z;
+    if (bouss) {
+      tval = a.state[2][idx];
+      bval = (a.bdir == 0 ? a.state[0] : a.state[1])[idx];
+    }
+    const cpx<T> e0 = nf > 2 ? stage[2][i] : z;
+    epi_mode(a, idx, g.kx[cl], g.ky[j], g.kz[slab], stage[0][i], stage[1][i], z, e0, z, z, tval,
+             bval);
+  }
+}
+
 }  // namespace ddx

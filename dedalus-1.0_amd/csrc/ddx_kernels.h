@@ -290,6 +290,7 @@ struct EpiArgs {
   const cpx<T>* upd_y[DDX_MAXF];
   const cpx<T>* upd_f1[DDX_MAXF];
   cpx<T>* upd_out[DDX_MAXF];
+  const cpx<T>* tw_y;   // twiddles of length Ny (fused forward-y epilogue)
 };
 
 template <typename T>
@@ -412,44 +413,65 @@ __global__ void update_kernel(const UpdArgs<T> a) {
   }
 }
 
-// k-space epilogue of the fused right-hand side: 1/N, forcing, buoyancy,
-// projection, induction curl; optionally the ETD sweep that follows it.
+// Per-mode pieces of the k-space epilogue, shared by the epilogue kernel and
+// the fused forward-y epilogue of small 2D grids.  The ETD sweep of
+// components v0 <= v < v1 of one mode (update_kernel's arithmetic): phi is
+// formed once per distinct coefficient.  The caller has read every state
+// entry of this mode that it still needs, so in-place outputs are safe.
 template <typename T>
-__global__ void epilogue_kernel(const EpiArgs<T> a) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.nk) return;
-  int col;
-  double kx, ky, kz;
-  kvec<double>(a.g, idx, col, kx, ky, kz);
+__device__ __forceinline__ void epi_etd(const EpiArgs<T>& a, long idx, double k2true, int v0, int v1,
+                                        const cpx<double> (&D)[6]) {
+  const int nd = a.ndim;
+  double kp;
+  if (a.upd_order == 1) kp = k2true;
+  else if (a.upd_order == 2) kp = k2true * k2true;
+  else kp = pow(k2true, (double)a.upd_order);
+  const double dt = a.upd_dt;
+  double cprev = 0.0, Z = 0.0, p0 = 1.0, p1 = 1.0, p2 = 0.5;
+  bool have = false;
+#pragma unroll
+  for (int v = 0; v < 6; ++v) {
+    if (v < v0 || v >= v1) continue;
+    const double c = a.upd_coeff[v];
+    if (!have || c != cprev) {
+      Z = (c != 0.0 && a.upd_kind != UPD_EULER) ? (-(c * kp)) * dt : 0.0;
+      if (Z != 0.0) phi012(Z, nd == 2, p0, p1, p2);
+      cprev = c;
+      have = true;
+    }
+    const cpx<double> dv = c64(cT<T>(D[v]));  // the value an unfused sweep would read back
+    const cpx<double> y = c64(a.upd_y[v][idx]);
+    const cpx<double> f1 = (a.upd_kind >= UPD_ETD2RK1) ? c64(a.upd_f1[v][idx]) : dv;
+    a.upd_out[v][idx] = cT<T>(etd_combine(a.upd_kind, y, f1, dv, Z, p0, p1, p2, dt));
+  }
+}
+
+// All components of one mode: F = NL / N + forcing (+ g alpha T along the
+// buoyancy axis), projected (F <- F - k (k.F) / |k|^2, |k|^2 = 0 replaced by
+// 1); Boussinesq T; MHD induction i k x E.  n0..n2: the unscaled forward
+// products of the velocity; e0..e2: those of T (e0), of E (3D) or of the
+// scalar E (2D, e0); tval, bval: the state's T and its velocity along the
+// buoyancy axis at this mode (Boussinesq), read by the caller before any
+// write of the mode.
+template <typename T>
+__device__ __forceinline__ void epi_mode(const EpiArgs<T>& a, long idx, double kx, double ky,
+                                         double kz, cpx<T> n0, cpx<T> n1, cpx<T> n2, cpx<T> e0_,
+                                         cpx<T> e1_, cpx<T> e2_, cpx<T> tval, cpx<T> bval) {
   const int nd = a.ndim;
   const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
   const bool upd = a.upd_kind >= 0;
   const bool wd = !upd || a.write_deriv;
-  if (col >= a.g.Kx) {
-#pragma unroll
-    for (int v = 0; v < 6; ++v) {
-      if (v >= nvar) break;
-      if (wd) a.deriv[v][idx] = mk(T(0), T(0));
-      if (upd) a.upd_out[v][idx] = mk(T(0), T(0));
-    }
-    return;
-  }
   const double s = 1.0 / a.npts;
   const double kk[3] = {kx, ky, kz};
-  // arrays picked by ndim / direction without dynamic indexing (keeps the
-  // argument arrays and locals out of scratch memory)
-  const cpx<T>* stT = a.physics == PH_BOUSS ? (nd == 2 ? a.state[2] : a.state[3]) : nullptr;
-  const cpx<T>* stB = a.physics == PH_BOUSS
-                          ? (a.bdir == 0 ? a.state[0] : (a.bdir == 1 ? a.state[1] : a.state[2]))
-                          : nullptr;
+  const cpx<T> nl[3] = {n0, n1, n2};
   cpx<double> F[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     F[i] = mk(0.0, 0.0);
     if (i < nd) {
-      F[i] = scale(c64(a.U[i][idx]), s);
+      F[i] = scale(c64(nl[i]), s);
       if (a.forcing[i]) F[i] = F[i] + c64(a.forcing[i][idx]);
-      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(c64(stT[idx]), a.galpha);
+      if (a.physics == PH_BOUSS && i == a.bdir) F[i] = F[i] + scale(c64(tval), a.galpha);
     }
   }
   // projection F <- F - k (k.F) / |k|^2, |k|^2 = 0 replaced by 1
@@ -466,22 +488,20 @@ __global__ void epilogue_kernel(const EpiArgs<T> a) {
   for (int i = 0; i < 3; ++i)
     if (i < nd) D[i] = F[i] - scale(kf, kk[i] / k2);
   if (a.physics == PH_BOUSS) {
-    const cpx<T>* UT = nd == 2 ? a.U[2] : a.U[3];
     const cpx<T>* fT = nd == 2 ? a.forcing[2] : a.forcing[3];
-    cpx<double> ft = scale(c64(UT[idx]), s) - scale(c64(stB[idx]), a.beta);
+    cpx<double> ft = scale(c64(e0_), s) - scale(c64(bval), a.beta);
     if (fT) ft = ft + c64(fT[idx]);
     if (nd == 2) D[2] = ft;
     else D[3] = ft;
   } else if (a.physics == PH_MHD) {
     if (nd == 3) {
-      const cpx<double> e0 = scale(c64(a.U[3][idx]), s), e1 = scale(c64(a.U[4][idx]), s),
-                        e2 = scale(c64(a.U[5][idx]), s);
+      const cpx<double> e0 = scale(c64(e0_), s), e1 = scale(c64(e1_), s), e2 = scale(c64(e2_), s);
       // i k x E
       D[3] = mul_is(e2, ky) - mul_is(e1, kz);
       D[4] = mul_is(e0, kz) - mul_is(e2, kx);
       D[5] = mul_is(e1, kx) - mul_is(e0, ky);
     } else {
-      const cpx<double> e = scale(c64(a.U[2][idx]), s);
+      const cpx<double> e = scale(c64(e0_), s);
       D[2] = mul_is(e, ky);
       D[3] = mk(0.0, 0.0) - mul_is(e, kx);
     }
@@ -491,31 +511,55 @@ __global__ void epilogue_kernel(const EpiArgs<T> a) {
     for (int v = 0; v < 6; ++v)
       if (v < nvar) a.deriv[v][idx] = cT<T>(D[v]);
   }
-  if (!upd) return;
-  // the ETD sweep of this mode (update_kernel's arithmetic; every state
-  // entry of this mode was read above, so in-place outputs are safe)
-  double kp;
-  if (a.upd_order == 1) kp = k2true;
-  else if (a.upd_order == 2) kp = k2true * k2true;
-  else kp = pow(k2true, (double)a.upd_order);
-  const double dt = a.upd_dt;
-  double cprev = 0.0, Z = 0.0, p0 = 1.0, p1 = 1.0, p2 = 0.5;
-  bool have = false;
+  if (upd) epi_etd(a, idx, k2true, 0, nvar, D);
+}
+
+// Zeros for the pad columns kx >= Kx of one mode (all components).
+template <typename T>
+__device__ __forceinline__ void epi_pad(const EpiArgs<T>& a, long idx, int nvar) {
+  const bool upd = a.upd_kind >= 0;
+  const bool wd = !upd || a.write_deriv;
 #pragma unroll
   for (int v = 0; v < 6; ++v) {
     if (v >= nvar) break;
-    const double c = a.upd_coeff[v];
-    if (!have || c != cprev) {
-      Z = (c != 0.0 && a.upd_kind != UPD_EULER) ? (-(c * kp)) * dt : 0.0;
-      if (Z != 0.0) phi012(Z, nd == 2, p0, p1, p2);
-      cprev = c;
-      have = true;
-    }
-    const cpx<double> dv = c64(cT<T>(D[v]));  // the value an unfused sweep would read back
-    const cpx<double> y = c64(a.upd_y[v][idx]);
-    const cpx<double> f1 = (a.upd_kind >= UPD_ETD2RK1) ? c64(a.upd_f1[v][idx]) : dv;
-    a.upd_out[v][idx] = cT<T>(etd_combine(a.upd_kind, y, f1, dv, Z, p0, p1, p2, dt));
+    if (wd) a.deriv[v][idx] = mk(T(0), T(0));
+    if (upd) a.upd_out[v][idx] = mk(T(0), T(0));
   }
+}
+
+// k-space epilogue of the fused right-hand side: 1/N, forcing, buoyancy,
+// projection, induction curl; optionally the ETD sweep that follows it.
+template <typename T>
+__global__ void epilogue_kernel(const EpiArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nk) return;
+  int col;
+  double kx, ky, kz;
+  kvec<double>(a.g, idx, col, kx, ky, kz);
+  const int nd = a.ndim;
+  const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
+  if (col >= a.g.Kx) {
+    epi_pad(a, idx, nvar);
+    return;
+  }
+  const cpx<T> z = mk(T(0), T(0));
+  // arrays picked by ndim / direction without dynamic indexing (keeps the
+  // argument arrays and locals out of scratch memory)
+  cpx<T> tval = z, bval = z;
+  if (a.physics == PH_BOUSS) {
+    tval = (nd == 2 ? a.state[2] : a.state[3])[idx];
+    bval = (a.bdir == 0 ? a.state[0] : (a.bdir == 1 ? a.state[1] : a.state[2]))[idx];
+  }
+  const cpx<T> n0 = a.U[0][idx], n1 = a.U[1][idx], n2 = nd == 3 ? a.U[2][idx] : z;
+  cpx<T> e0 = z, e1 = z, e2 = z;
+  if (nvar > nd) {
+    e0 = (nd == 2 ? a.U[2] : a.U[3])[idx];
+    if (a.physics == PH_MHD && nd == 3) {
+      e1 = a.U[4][idx];
+      e2 = a.U[5][idx];
+    }
+  }
+  epi_mode(a, idx, kx, ky, kz, n0, n1, n2, e0, e1, e2, tval, bval);
 }
 
 // layout conversion: kref (reference transposed half spectrum) <-> compact

@@ -353,6 +353,46 @@ def test_steps_with_constant_forcing(shape, integ):
             assert rel_l2(got['u'][c], ref['u'][c]) < 1e-12, (scale, c)
 
 
+@pytest.mark.parametrize("kind,shape,integ", [('hydro', (64, 64), 'RK2mid'),
+                                                ('boussinesq', (64, 128), 'RK2trap'),
+                                                ('mhd', (64, 48), 'RK2mid'),
+                                                ('hydro', (64, 64), 'RK2trap')], ids=str)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_fused_y_epilogue_matches_separate(kind, shape, integ, precision):
+    """2D steps on 64-point y axes run the forward y pass inside the epilogue;
+    the states equal those of separate launches (DDX_NO_FUSED_YEPI=1) bit for
+    bit, and stay within the oracle's tolerance."""
+    import os
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("fused y epilogue", kind, shape, integ, precision)
+    prec = hp.F64 if precision == "f64" else hp.F32
+    p = {'nu': 2e-3, 'kappa': 1e-3, 'eta': 3e-3, 'boussinesq_direction': 'y'}
+    kf = kfields(kind, shape, 75, L)
+    integrator = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+    out = []
+    for sep in (False, True):
+        if sep:
+            os.environ["DDX_NO_FUSED_YEPI"] = "1"
+        try:
+            plan = hp.Plan(shape, L, prec)
+        finally:
+            os.environ.pop("DDX_NO_FUSED_YEPI", None)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        plan.step(integrator, params_struct(hp, kind, nd, p), st, bufs[0], bufs[1], bufs[2], 4, 5e-3)
+        out.append(download_fields(plan, kind, nd, st))
+    for name in FIELDS[kind]:
+        for c in range(len(out[0][name])):
+            assert np.array_equal(out[0][name][c], out[1][name][c]), (name, c)
+    if precision == "f64":
+        ref = npth.run(kind, shape, kf, 5e-3, 4, integ, L, **dict(p))
+        for name in FIELDS[kind]:
+            for c in range(len(ref[name])):
+                assert rel_l2(out[0][name][c], ref[name][c]) < 1e-12, (name, c)
+
+
 FUSE_CASES = [('hydro', (32, 32), 'RK2mid'), ('hydro', (16, 16, 16), 'RK2trap'),
               ('boussinesq', (32, 48), 'RK2mid'), ('boussinesq', (16, 8, 16), 'RK2trap'),
               ('mhd', (16, 16, 16), 'RK2mid'), ('mhd', (32, 32), 'RK2trap'),
