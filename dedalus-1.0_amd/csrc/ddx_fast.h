@@ -571,16 +571,18 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
 // First-pass inputs of one column: the stored rows (compact layouts leave out
 // the rows the 2/3 rule removes), or with SPECTRAL the recipe's NT terms
 // coef * (i k)^[dim >= 0] * src evaluated on the fly.  Every load is issued
-// unconditionally, from a clamped in-range address (the last column for the
-// lanes past it, which lies in the same tile and lines; row 0 for rows left
-// out), and unused values are zeroed by a select
+// unconditionally, from a clamped in-range address (a column of the previous
+// tile for the lanes past the last column; row 0 for rows left out), and
+// unused values are zeroed by a select
 // afterwards: loads under lane-dependent or per-term branches each end in a
 // wait for memory, which serialises the column's loads.
-template <typename T, int N, class PL, bool SPECTRAL, int NT>
+template <typename T, int N, class PL, bool SPECTRAL, int NT, int W>
 __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>& a, int lane, int f,
                                            long slab, int col, bool live) {
   constexpr int LANES = PL::LANES, E = PL::E, R0 = PL::R[0];
-  const int colc = live ? col : a.ncols - 1;
+  // lanes past the last column read the same lane's column of the previous
+  // tile: distinct addresses in lines that tile's workgroup has just read
+  const int colc = live ? col : (col >= 2 * W ? col - W : a.ncols - 1);
   // per term: source, coefficient and wavenumber table (the kx table, valid
   // in every pass, stands in for terms without a derivative)
   constexpr int NTA = NT > 0 ? NT : 1;
@@ -687,10 +689,10 @@ col_fft_fast(const ColArgs<T> a) {
   if (SPECTRAL) {
     // the term count is uniform: one straight-line gather per count
     const int nterm = a.rec[f].nterm;
-    if (nterm >= 2) col_gather<T, N, PL, true, 2>(v, a, lane, f, slab, col, live);
-    else col_gather<T, N, PL, true, 1>(v, a, lane, f, slab, col, live);
+    if (nterm >= 2) col_gather<T, N, PL, true, 2, W>(v, a, lane, f, slab, col, live);
+    else col_gather<T, N, PL, true, 1, W>(v, a, lane, f, slab, col, live);
   } else {
-    col_gather<T, N, PL, false, 0>(v, a, lane, f, slab, col, live);
+    col_gather<T, N, PL, false, 0, W>(v, a, lane, f, slab, col, live);
   }
   if constexpr (SPLIT) run_passes_split<T, D, N, PL, false, W, POW>(v, lane, c, lds, twc);
   else run_passes_cached<T, D, N, PL, false, W, POW>(v, lane, c, lds, twc);
