@@ -18,6 +18,12 @@
 #ifndef DDX_COL_XCD_GROUP
 #define DDX_COL_XCD_GROUP 1
 #endif
+#ifndef DDX_YSPEC_GROUP
+#define DDX_YSPEC_GROUP 1
+#endif
+#ifndef DDX_YSPEC_WIDE
+#define DDX_YSPEC_WIDE 1
+#endif
 #include "ddx_fast.h"
 
 using namespace ddx;
@@ -131,6 +137,10 @@ struct ddx_plan {
   // single 2D grids with 64-point y axes run the forward y pass and the
   // k-space epilogue as one kernel (DDX_NO_FUSED_YEPI=1 runs them as two)
   bool fuse_yepi = true;
+  // FP64 3D right-hand sides transform along z only the arrays whose
+  // derivatives need k_z, and form the other derived fields in the y pass
+  // (DDX_NO_ZSPLIT=1 forms every derived field before the z pass)
+  bool zsplit = true;
   int nbatch = 1;
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
@@ -236,31 +246,41 @@ int tile_width(int n, size_t cb) {
 
 enum ColKind { COL_PLAIN, COL_SPECTRAL };
 
-// Size-specialised column kernel for n in {64, 128, 256, 512}; false if none.
+template <typename T, int N, bool WIDE>
+void col_fast_go(ddx_plan* p, const ColArgs<T>& a, dim3 grid, int dir, bool spectral) {
+  const dim3 block(ColPlan<N>::LANES * col_width<N, WIDE>());
+  if (dir > 0) {
+    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true, WIDE>), grid, block, 0, p->stream, a);
+    else hipLaunchKernelGGL((col_fft_fast<T, 1, N, false, WIDE>), grid, block, 0, p->stream, a);
+  } else {
+    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, -1, N, true, WIDE>), grid, block, 0, p->stream, a);
+    else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false, WIDE>), grid, block, 0, p->stream, a);
+  }
+}
+
+// Size-specialised column kernel for n in {64, 128, 256, 512}
 template <typename T, int N>
 void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
-  using PL = ColPlan<N>;
-  const int W = spectral ? col_width<T, N, true>() : col_width<T, N, false>();
+  // double-width tiles at n = 256: FP32, the plain FP64 passes, and the
+  // spectral passes whose caller asks for them
+  const bool wide = N == 256 && (sizeof(T) == 4 || !spectral || a.wide);
+  const int W = wide ? col_width<N, true>() : col_width<N, false>();
   const int ntiles = (a.ncols + W - 1) / W;
   dim3 grid(ntiles, nslabs, nfields);
-  dim3 block(PL::LANES * W);
   a.nslabs = nslabs;
   a.nfields = nfields;
   a.ntiles = ntiles;
   // The 3D spectral z pass reads each source for up to three fields: group
   // them per XCD (11 % faster at 256^3; no gain measured on the 2D y pass)
-  a.xcd_group = spectral && nfields > 1 && a.row_dim == 2 && DDX_COL_XCD_GROUP;
+  a.xcd_group = spectral && nfields > 1 && (a.row_dim == 2 || a.group_hint) && DDX_COL_XCD_GROUP;
   if (a.xcd_group) {
     const long units = (long)ntiles * nslabs;
     grid = dim3((unsigned)(((units + 7) / 8) * 8 * nfields), 1, 1);
   }
-  if (dir > 0) {
-    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true>), grid, block, 0, p->stream, a);
-    else hipLaunchKernelGGL((col_fft_fast<T, 1, N, false>), grid, block, 0, p->stream, a);
-  } else {
-    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, -1, N, true>), grid, block, 0, p->stream, a);
-    else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false>), grid, block, 0, p->stream, a);
+  if constexpr (N == 256) {
+    if (wide) { col_fast_go<T, N, true>(p, a, grid, dir, spectral); return; }
   }
+  col_fast_go<T, N, false>(p, a, grid, dir, spectral);
 }
 
 template <typename T>
@@ -606,6 +626,30 @@ int phys_layout(int physics, int nd, PhysLayout* L) {
   return 0;
 }
 
+// The backward z pass of a 3D right-hand side splits the derived fields:
+// a derivative along z must be taken before the z transform, but one along x
+// or y can wait for the y pass.  Each distinct (source, with or without d/dz)
+// of the recipes becomes one z-pass output, and the y pass forms the derived
+// fields from those outputs with the remaining x and y derivatives.  MHD then
+// transforms 10 arrays along z instead of 12 (u, dz ux, dz uy and the same
+// for B), hydro 5 instead of 6.  Returns the number of z-pass outputs.
+int split_z(const PhysLayout& L, Recipe* zrec, Recipe* yrec) {
+  int nz = 0;
+  for (int q = 0; q < L.nq; ++q) {
+    yrec[q] = L.rec[q];
+    for (int t = 0; t < L.rec[q].nterm; ++t) {
+      const Term tm = L.rec[q].t[t];
+      const int zd = tm.dim == 2 ? 2 : -1;
+      int j = 0;
+      while (j < nz && !(zrec[j].t[0].src == tm.src && zrec[j].t[0].dim == zd)) ++j;
+      if (j == nz) zrec[nz++] = one(tm.src, zd);
+      yrec[q].t[t].src = j;
+      yrec[q].t[t].dim = tm.dim == 2 ? -1 : tm.dim;
+    }
+  }
+  return nz;
+}
+
 int nstate_of(const ddx_params* prm, int nd) {
   PhysLayout L;
   if (phys_layout(prm->physics, nd, &L)) return -1;
@@ -647,7 +691,24 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int s = 0; s < L.nstate; ++s) c.src[s] = state[s];
   for (int q = 0; q < L.nq; ++q) c.rec[q] = L.rec[q];
   cpx<T>* mixed;  // (Nz, Ny, Kxp) per field
-  if (p->zfft) {
+  // FP64 only: at 256^3 MHD the z pass gets 25 % faster (10 one-term
+  // outputs instead of 12 with two-term curls) and the y pass, which then
+  // forms the curls, 8 % slower: 1.7 % per step.  FP32 at 128^3 measured
+  // 6 % slower per step (its y pass is short of load issue, not bytes).
+  Recipe zrec[DDX_MAXF], yrec[DDX_MAXF];
+  const int nzs = p->zfft && p->zsplit && sizeof(T) == 8 ? split_z(L, zrec, yrec) : L.nq;
+  if (p->zfft && nzs < L.nq) {
+    geom_bz(p, c);
+    for (int j = 0; j < nzs; ++j) { c.rec[j] = zrec[j]; c.out[j] = A + j * fs; }
+    if ((rc = launch_col(p, c, p->Ky, nzs, 1, true, PC_COL_BWD_Z_SPECTRAL, L.nstate))) return rc;
+    col_defaults(c);
+    geom_by(p, c);
+    for (int j = 0; j < nzs; ++j) c.src[j] = A + j * fs;
+    for (int q = 0; q < L.nq; ++q) { c.rec[q] = yrec[q]; c.out[q] = B + q * fs; }
+    c.group_hint = DDX_YSPEC_GROUP;
+    c.wide = DDX_YSPEC_WIDE;
+    if ((rc = launch_col(p, c, p->az.n, L.nq, 1, true, PC_COL_BWD_Y_SPECTRAL, nzs))) return rc;
+  } else if (p->zfft) {
     geom_bz(p, c);
     for (int q = 0; q < L.nq; ++q) c.out[q] = A + q * fs;
     if ((rc = launch_col(p, c, p->Ky, L.nq, 1, true, PC_COL_BWD_Z_SPECTRAL, L.nstate))) return rc;
@@ -978,6 +1039,8 @@ int plan_create(int ndim, const int64_t* xshape, const double* length, int preci
     p->fuse_update = !(nf && nf[0] == '1');
     const char* ny = getenv("DDX_NO_FUSED_YEPI");
     p->fuse_yepi = !(ny && ny[0] == '1');
+    const char* nz = getenv("DDX_NO_ZSPLIT");
+    p->zsplit = !(nz && nz[0] == '1');
   }
   p->ndim = ndim;
   p->precision = precision;

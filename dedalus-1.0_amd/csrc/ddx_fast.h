@@ -32,14 +32,14 @@ template <> struct ColPlan<128> { static constexpr int LANES = 8,  E = 16, W = 1
 template <> struct ColPlan<256> { static constexpr int LANES = 16, E = 16, W = 8,  NP = 2; static constexpr int R[2] = {16, 16}; };
 template <> struct ColPlan<512> { static constexpr int LANES = 64, E = 8,  W = 4,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
 
-// Column tile width of the kernel for element type T: at n = 256, FP32 tiles
-// are 16 columns wide, so a tile row is a whole 128-byte line, and FP64
-// tiles of the plain passes too (256-byte rows; their exchanges then go
-// through the half-size real buffer): 6 % faster y passes at 256^3.  The
-// spectral pass keeps 8 FP64 columns, where 16 were slower.
-template <typename T, int N, bool SPECTRAL>
+// Column tiles of double width (WIDE, n = 256 only): FP32 tiles are then 16
+// columns wide, so a tile row is a whole 128-byte line, and FP64 tiles of
+// the plain passes too (256-byte rows; their exchanges then go through the
+// half-size real buffer): 6 % faster y passes at 256^3.  The FP64 spectral
+// z pass keeps 8 columns, where 16 were slower.
+template <int N, bool WIDE>
 constexpr int col_width() {
-  return ColPlan<N>::W * ((N == 256 && (sizeof(T) == 4 || !SPECTRAL)) ? 2 : 1);
+  return ColPlan<N>::W * (WIDE ? 2 : 1);
 }
 
 // LDS index of position m in the exchange written by a pass of radix R after
@@ -573,35 +573,40 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
 // coef * (i k)^[dim >= 0] * src evaluated on the fly.  Every load is issued
 // unconditionally, from a clamped in-range address (a column of the previous
 // tile for the lanes past the last column; row 0 for rows left out), and
-// unused values are zeroed by a select
-// afterwards: loads under lane-dependent or per-term branches each end in a
-// wait for memory, which serialises the column's loads.
-template <typename T, int N, class PL, bool SPECTRAL, int NT, int W>
+// unused values are zeroed by a select afterwards: loads under lane-dependent
+// or per-term branches each end in a wait for memory, which serialises the
+// column's loads.  Wavenumbers: k_x is one load per lane (the column), the
+// slab axis's one per workgroup, and the row axis's one per element, loaded
+// only by the ROWK variant (a term differentiates along the row axis) and
+// shared by both terms.
+template <typename T, int N, class PL, bool SPECTRAL, int NT, bool ROWK, int W>
 __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>& a, int lane, int f,
                                            long slab, int col, bool live) {
   constexpr int LANES = PL::LANES, E = PL::E, R0 = PL::R[0];
   // lanes past the last column read the same lane's column of the previous
   // tile: distinct addresses in lines that tile's workgroup has just read
   const int colc = live ? col : (col >= 2 * W ? col - W : a.ncols - 1);
-  // per term: source, coefficient and wavenumber table (the kx table, valid
-  // in every pass, stands in for terms without a derivative)
   constexpr int NTA = NT > 0 ? NT : 1;
   const cpx<T>* srcp[NTA];
-  const double* kp[NTA];
   bool deriv[NTA], byrow[NTA], byslab[NTA];
   T coef[NTA];
+  double kc = 0.0, ks = 0.0;
+  if (SPECTRAL) {
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const Term tm = a.rec[f].t[t];
-    srcp[t] = a.src[tm.src];
-    deriv[t] = tm.dim >= 0;
-    byrow[t] = tm.dim > 0 && tm.dim == a.row_dim;
-    byslab[t] = tm.dim > 0 && tm.dim != a.row_dim;
-    kp[t] = byrow[t] ? a.k_row : (byslab[t] ? a.k_slab : a.k_col);
-    coef[t] = T(tm.coef);
+    for (int t = 0; t < NT; ++t) {
+      const Term tm = a.rec[f].t[t];
+      srcp[t] = a.src[tm.src];
+      deriv[t] = tm.dim >= 0;
+      byrow[t] = tm.dim > 0 && tm.dim == a.row_dim;
+      byslab[t] = tm.dim > 0 && tm.dim != a.row_dim;
+      coef[t] = T(tm.coef);
+    }
+    kc = a.k_col[colc];
+    // passes without a slab table read k_x[0] in its place (never used)
+    ks = (a.k_slab ? a.k_slab : a.k_col)[a.k_slab ? slab : 0];
   }
   cpx<T> ld[NTA][E];
-  double kv[NTA][E];
+  double kr[ROWK ? E : 1];
   bool ok[E];
 #pragma unroll
   for (int qq = 0; qq < E / R0; ++qq)
@@ -619,10 +624,8 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
       const long off = slab * a.in_ss + (long)rowc * a.in_rs + colc;
       if (SPECTRAL) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          ld[t][e] = srcp[t][off];
-          kv[t][e] = kp[t][byrow[t] ? (long)rowc : (byslab[t] ? slab : (long)colc)];
-        }
+        for (int t = 0; t < NT; ++t) ld[t][e] = srcp[t][off];
+        if (ROWK) kr[e] = a.k_row[rowc];
       } else {
         ld[0][e] = a.in[f][off];
       }
@@ -633,7 +636,8 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
     if (SPECTRAL) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const cpx<T> sv = deriv[t] ? mul_is(ld[t][e], T(kv[t][e])) : ld[t][e];
+        const double kk = (ROWK && byrow[t]) ? kr[ROWK ? e : 0] : (byslab[t] ? ks : kc);
+        const cpx<T> sv = deriv[t] ? mul_is(ld[t][e], T(kk)) : ld[t][e];
         x = x + scale(sv, coef[t]);
       }
     } else {
@@ -643,11 +647,11 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
   }
 }
 
-template <typename T, int D, int N, bool SPECTRAL>
-__global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<T, N, SPECTRAL>()))
+template <typename T, int D, int N, bool SPECTRAL, bool WIDE>
+__global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<N, WIDE>()))
 col_fft_fast(const ColArgs<T> a) {
   using PL = ColPlan<N>;
-  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<T, N, SPECTRAL>();
+  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<N, WIDE>();
   constexpr int R0 = PL::R[0];
   constexpr int RL = PL::R[PL::NP - 1];
   constexpr int BUF = exch_size<N, PL>();
@@ -688,11 +692,19 @@ col_fft_fast(const ColArgs<T> a) {
   cpx<T> v[E];
   if (SPECTRAL) {
     // the term count is uniform: one straight-line gather per count
-    const int nterm = a.rec[f].nterm;
-    if (nterm >= 2) col_gather<T, N, PL, true, 2, W>(v, a, lane, f, slab, col, live);
-    else col_gather<T, N, PL, true, 1, W>(v, a, lane, f, slab, col, live);
+    const Recipe& rc = a.rec[f];
+    const int nterm = rc.nterm;
+    const bool rowk = (rc.t[0].dim > 0 && rc.t[0].dim == a.row_dim) ||
+                      (nterm >= 2 && rc.t[1].dim > 0 && rc.t[1].dim == a.row_dim);
+    if (nterm >= 2) {
+      if (rowk) col_gather<T, N, PL, true, 2, true, W>(v, a, lane, f, slab, col, live);
+      else col_gather<T, N, PL, true, 2, false, W>(v, a, lane, f, slab, col, live);
+    } else {
+      if (rowk) col_gather<T, N, PL, true, 1, true, W>(v, a, lane, f, slab, col, live);
+      else col_gather<T, N, PL, true, 1, false, W>(v, a, lane, f, slab, col, live);
+    }
   } else {
-    col_gather<T, N, PL, false, 0, W>(v, a, lane, f, slab, col, live);
+    col_gather<T, N, PL, false, 0, false, W>(v, a, lane, f, slab, col, live);
   }
   if constexpr (SPLIT) run_passes_split<T, D, N, PL, false, W, POW>(v, lane, c, lds, twc);
   else run_passes_cached<T, D, N, PL, false, W, POW>(v, lane, c, lds, twc);

@@ -52,6 +52,8 @@ struct ColArgs {
   int nslabs, nfields;   // grid extents (used by the remapped 1D grids)
   int ntiles;            // column tiles per slab
   int xcd_group;         // 1D grid: the fields of one (tile, slab) on one XCD
+  int group_hint;        // set by the caller: group the fields per XCD (spectral passes)
+  int wide;              // set by the caller: double-width tiles (n = 256)
   T scale;
 };
 

@@ -393,6 +393,53 @@ def test_fused_y_epilogue_matches_separate(kind, shape, integ, precision):
                 assert rel_l2(out[0][name][c], ref[name][c]) < 1e-12, (name, c)
 
 
+ZSPLIT_CASES = [('mhd', (64, 64, 64), {}), ('hydro', (32, 256, 64), {}),
+                ('boussinesq', (256, 32, 32), {'boussinesq_direction': 'z'}),
+                ('boussinesq', (16, 256, 16), {'boussinesq_direction': 'x'}),
+                ('mhd', (12, 20, 24), {}), ('hydro', (10, 8, 6), {})]
+
+
+@pytest.mark.parametrize("kind,shape,extra", ZSPLIT_CASES, ids=str)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_z_split_matches_unsplit(kind, shape, extra, precision):
+    """FP64 3D right-hand sides transform along z only u, dz ux, dz uy (and
+    B, T, dz T) and form the curls and gradients in the y pass.  The states
+    agree with the unsplit dataflow (DDX_NO_ZSPLIT=1, every derived field
+    formed before the z pass) to rounding, and with the oracle within 1e-12.
+    FP32 plans do not split: the two runs are then identical."""
+    import os
+    hp = hip()
+    L = LENGTHS[3]
+    case("z split", kind, shape, precision)
+    prec = hp.F64 if precision == "f64" else hp.F32
+    p = {'nu': 2e-3, 'kappa': 1e-3, 'eta': 3e-3}
+    p.update(extra)
+    kf = kfields(kind, shape, 91, L)
+    out = []
+    for unsplit in (False, True):
+        if unsplit:
+            os.environ["DDX_NO_ZSPLIT"] = "1"
+        try:
+            plan = hp.Plan(shape, L, prec)
+        finally:
+            os.environ.pop("DDX_NO_ZSPLIT", None)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        plan.step(hp.RK2MID, params_struct(hp, kind, 3, p), st, bufs[0], bufs[1], bufs[2], 3, 2e-3)
+        out.append(download_fields(plan, kind, 3, st))
+    for name in FIELDS[kind]:
+        for c in range(len(out[0][name])):
+            if precision == "f64":
+                assert rel_l2(out[0][name][c], out[1][name][c]) < 1e-14, (name, c)
+            else:
+                assert np.array_equal(out[0][name][c], out[1][name][c]), (name, c)
+    if precision == "f64":
+        ref = npth.run(kind, shape, kf, 2e-3, 3, 'RK2mid', L, **dict(p))
+        for name in FIELDS[kind]:
+            for c in range(len(ref[name])):
+                assert rel_l2(out[0][name][c], ref[name][c]) < 1e-12, (name, c)
+
+
 FUSE_CASES = [('hydro', (32, 32), 'RK2mid'), ('hydro', (16, 16, 16), 'RK2trap'),
               ('boussinesq', (32, 48), 'RK2mid'), ('boussinesq', (16, 8, 16), 'RK2trap'),
               ('mhd', (16, 16, 16), 'RK2mid'), ('mhd', (32, 32), 'RK2trap'),
