@@ -628,13 +628,48 @@ int phys_layout(int physics, int nd, PhysLayout* L) {
 
 // The backward z pass of a 3D right-hand side splits the derived fields:
 // a derivative along z must be taken before the z transform, but one along x
-// or y can wait for the y pass.  Each distinct (source, with or without d/dz)
-// of the recipes becomes one z-pass output, and the y pass forms the derived
-// fields from those outputs with the remaining x and y derivatives.  MHD then
-// transforms 10 arrays along z instead of 12 (u, dz ux, dz uy and the same
-// for B), hydro 5 instead of 6.  Returns the number of z-pass outputs.
+// or y can wait for the y pass.
+// DDX_ZSPLIT_MODE 1: each distinct (source, with or without d/dz) of the
+// recipes becomes one z-pass output, and the y pass forms every derived field
+// from those with the remaining x and y derivatives (MHD: u, dz ux, dz uy
+// and the same for B; every curl is a two-term y-pass field).
+// DDX_ZSPLIT_MODE 2: fields with a d/dz term, and plain copies, are formed in
+// the z pass; fields without one wait for the y pass and are formed there
+// from plain z-pass outputs (MHD: u, wx, wy and the same for B; wz and Jz are
+// the only two-term y-pass fields).
+// Either way MHD transforms 10 arrays along z instead of 12, hydro 5 instead
+// of 6.  Returns the number of z-pass outputs.
+#ifndef DDX_ZSPLIT_MODE
+#define DDX_ZSPLIT_MODE 2
+#endif
 int split_z(const PhysLayout& L, Recipe* zrec, Recipe* yrec) {
   int nz = 0;
+  // index of the z-pass output that is source s, plain (added if missing)
+  auto plain = [&](int s) {
+    int j = 0;
+    while (j < nz && !(zrec[j].nterm == 1 && zrec[j].t[0].src == s && zrec[j].t[0].dim < 0 &&
+                       zrec[j].t[0].coef == 1.0)) ++j;
+    if (j == nz) zrec[nz++] = one(s);
+    return j;
+  };
+  if (DDX_ZSPLIT_MODE == 2) {
+    int zi[DDX_MAXF];
+    for (int q = 0; q < L.nq; ++q) {
+      const Recipe& r = L.rec[q];
+      bool dz = false;
+      for (int t = 0; t < r.nterm; ++t) dz = dz || r.t[t].dim == 2;
+      const bool copy = r.nterm == 1 && r.t[0].dim < 0 && r.t[0].coef == 1.0;
+      zi[q] = -1;
+      if (copy) zi[q] = plain(r.t[0].src);
+      else if (dz) { zi[q] = nz; zrec[nz++] = r; }
+    }
+    for (int q = 0; q < L.nq; ++q) {
+      if (zi[q] >= 0) { yrec[q] = one(zi[q]); continue; }
+      yrec[q] = L.rec[q];
+      for (int t = 0; t < L.rec[q].nterm; ++t) yrec[q].t[t].src = plain(L.rec[q].t[t].src);
+    }
+    return nz;
+  }
   for (int q = 0; q < L.nq; ++q) {
     yrec[q] = L.rec[q];
     for (int t = 0; t < L.rec[q].nterm; ++t) {
@@ -691,10 +726,10 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int s = 0; s < L.nstate; ++s) c.src[s] = state[s];
   for (int q = 0; q < L.nq; ++q) c.rec[q] = L.rec[q];
   cpx<T>* mixed;  // (Nz, Ny, Kxp) per field
-  // FP64 only: at 256^3 MHD the z pass gets 25 % faster (10 one-term
-  // outputs instead of 12 with two-term curls) and the y pass, which then
-  // forms the curls, 8 % slower: 1.7 % per step.  FP32 at 128^3 measured
-  // 6 % slower per step (its y pass is short of load issue, not bytes).
+  // FP64 only: at 256^3 MHD the z pass gets 26 % faster (10 outputs
+  // instead of 12) and the y pass 1 % faster: 2.5 % per step.  FP32 at
+  // 128^3 measured 6 % slower per step (its y pass is short of load issue,
+  // not bytes).
   Recipe zrec[DDX_MAXF], yrec[DDX_MAXF];
   const int nzs = p->zfft && p->zsplit && sizeof(T) == 8 ? split_z(L, zrec, yrec) : L.nq;
   if (p->zfft && nzs < L.nq) {
