@@ -665,11 +665,21 @@ col_fft_fast(const ColArgs<T> a) {
   const int c = threadIdx.x % W, lane = threadIdx.x / W;
   int f, tile;
   long slab;
-  if (a.xcd_group) {
-    // Workgroups are dealt to the 8 XCDs round robin by linear index L, so
+  if (a.xcd_group == 2) {
+    // Workgroups are dealt to the 8 XCDs round robin by linear index L.  XCD
+    // x = L mod 8 takes slabs x, x + 8, ..., and walks each slab tile by
+    // tile, the F fields of a tile one after another: fields that read the
+    // same sources (a spectral prologue) find them in that XCD's L2, and so
+    // do the lanes past the last column, which read the previous tile.
+    const long L = blockIdx.x, t = L >> 3;
+    const long per = (long)a.ntiles * a.nfields;
+    slab = (t / per) * 8 + (L & 7);
+    if (slab >= a.nslabs) return;   // padding of the last group of 8 slabs
+    tile = (int)((t % per) / a.nfields);
+    f = (int)(t % a.nfields);
+  } else if (a.xcd_group) {
     // L = 8 (floor(U / 8) F + f) + U mod 8 puts the F fields of unit
-    // U = (slab, tile) on one XCD, one after another: fields that read the
-    // same sources (a spectral prologue) then find them in that XCD's L2.
+    // U = (slab, tile) on one XCD, one after another
     const long L = blockIdx.x, t = L >> 3;
     const long U = (t / a.nfields) * 8 + (L & 7);
     if (U >= (long)a.ntiles * a.nslabs) return;   // padding of the last group

@@ -51,7 +51,7 @@ struct ColArgs {
   int row_dim, slab_dim; // physical axis (0 x, 1 y, 2 z) of rows / slabs, -1 none
   int nslabs, nfields;   // grid extents (used by the remapped 1D grids)
   int ntiles;            // column tiles per slab
-  int xcd_group;         // 1D grid: the fields of one (tile, slab) on one XCD
+  int xcd_group;         // 1D grid on XCDs: 1 the fields of a (slab, tile), 2 whole slabs
   int group_hint;        // set by the caller: group the fields per XCD (spectral passes)
   int wide;              // set by the caller: double-width tiles (n = 256)
   T scale;
