@@ -18,6 +18,9 @@
 #ifndef DDX_COL_XCD_GROUP
 #define DDX_COL_XCD_GROUP 1
 #endif
+#ifndef DDX_EPI_PRELOAD
+#define DDX_EPI_PRELOAD 1
+#endif
 #ifndef DDX_YSPEC_GROUP
 #define DDX_YSPEC_GROUP 1
 #endif
@@ -853,7 +856,10 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   }
   ProfScope prof(p, PC_EPILOGUE,
                  kvalid * (L.np + L.nstate + extra + nforce + upd_arrays * L.nstate) * sizeof(cpx<T>));
-  hipLaunchKernelGGL((epilogue_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
+  if (DDX_EPI_PRELOAD && fu && prm->physics == DDX_MHD && nd == 3)
+    hipLaunchKernelGGL((epilogue_kernel<T, true>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
+  else
+    hipLaunchKernelGGL((epilogue_kernel<T, false>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, e);
   LAUNCHCHK();
   return 0;
 }

@@ -422,7 +422,8 @@ __global__ void update_kernel(const UpdArgs<T> a) {
 // entry of this mode that it still needs, so in-place outputs are safe.
 template <typename T>
 __device__ __forceinline__ void epi_etd(const EpiArgs<T>& a, long idx, double k2true, int v0, int v1,
-                                        const cpx<double> (&D)[6]) {
+                                        const cpx<double> (&D)[6], const cpx<T>* yv = nullptr,
+                                        const cpx<T>* fv = nullptr) {
   const int nd = a.ndim;
   double kp;
   if (a.upd_order == 1) kp = k2true;
@@ -442,8 +443,9 @@ __device__ __forceinline__ void epi_etd(const EpiArgs<T>& a, long idx, double k2
       have = true;
     }
     const cpx<double> dv = c64(cT<T>(D[v]));  // the value an unfused sweep would read back
-    const cpx<double> y = c64(a.upd_y[v][idx]);
-    const cpx<double> f1 = (a.upd_kind >= UPD_ETD2RK1) ? c64(a.upd_f1[v][idx]) : dv;
+    // yv / fv: the mode's y and F1, read by the caller before any write
+    const cpx<double> y = c64(yv ? yv[v] : a.upd_y[v][idx]);
+    const cpx<double> f1 = (a.upd_kind >= UPD_ETD2RK1) ? c64(fv ? fv[v] : a.upd_f1[v][idx]) : dv;
     a.upd_out[v][idx] = cT<T>(etd_combine(a.upd_kind, y, f1, dv, Z, p0, p1, p2, dt));
   }
 }
@@ -458,7 +460,8 @@ __device__ __forceinline__ void epi_etd(const EpiArgs<T>& a, long idx, double k2
 template <typename T>
 __device__ __forceinline__ void epi_mode(const EpiArgs<T>& a, long idx, double kx, double ky,
                                          double kz, cpx<T> n0, cpx<T> n1, cpx<T> n2, cpx<T> e0_,
-                                         cpx<T> e1_, cpx<T> e2_, cpx<T> tval, cpx<T> bval) {
+                                         cpx<T> e1_, cpx<T> e2_, cpx<T> tval, cpx<T> bval,
+                                         const cpx<T>* yv = nullptr, const cpx<T>* fv = nullptr) {
   const int nd = a.ndim;
   const int nvar = nd + (a.physics == PH_BOUSS ? 1 : (a.physics == PH_MHD ? nd : 0));
   const bool upd = a.upd_kind >= 0;
@@ -513,7 +516,7 @@ __device__ __forceinline__ void epi_mode(const EpiArgs<T>& a, long idx, double k
     for (int v = 0; v < 6; ++v)
       if (v < nvar) a.deriv[v][idx] = cT<T>(D[v]);
   }
-  if (upd) epi_etd(a, idx, k2true, 0, nvar, D);
+  if (upd) epi_etd(a, idx, k2true, 0, nvar, D, yv, fv);
 }
 
 // Zeros for the pad columns kx >= Kx of one mode (all components).
@@ -531,7 +534,12 @@ __device__ __forceinline__ void epi_pad(const EpiArgs<T>& a, long idx, int nvar)
 
 // k-space epilogue of the fused right-hand side: 1/N, forcing, buoyancy,
 // projection, induction curl; optionally the ETD sweep that follows it.
-template <typename T>
+// PRE (3D MHD with a fused ETD sweep): the sweep's y and F1 are read with
+// the products, so that every load of a mode is issued before its first
+// store; otherwise the compiler, which cannot rule out that the deriv and
+// out arrays alias y and F1, issues them after the deriv stores and between
+// the per-component out stores.
+template <typename T, bool PRE>
 __global__ void epilogue_kernel(const EpiArgs<T> a) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.nk) return;
@@ -561,7 +569,20 @@ __global__ void epilogue_kernel(const EpiArgs<T> a) {
       e2 = a.U[5][idx];
     }
   }
-  epi_mode(a, idx, kx, ky, kz, n0, n1, n2, e0, e1, e2, tval, bval);
+  if (PRE) {
+    // F1 exists for the ETD2RK sweeps only; the other sweeps read y twice
+    // rather than branch around a load
+    const bool hasf = a.upd_kind >= UPD_ETD2RK1;
+    cpx<T> yv[6], fv[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+      yv[v] = a.upd_y[v][idx];
+      fv[v] = (hasf ? a.upd_f1[v] : a.upd_y[v])[idx];
+    }
+    epi_mode(a, idx, kx, ky, kz, n0, n1, n2, e0, e1, e2, tval, bval, yv, fv);
+  } else {
+    epi_mode(a, idx, kx, ky, kz, n0, n1, n2, e0, e1, e2, tval, bval);
+  }
 }
 
 // layout conversion: kref (reference transposed half spectrum) <-> compact

@@ -443,7 +443,8 @@ def test_z_split_matches_unsplit(kind, shape, extra, precision):
 FUSE_CASES = [('hydro', (32, 32), 'RK2mid'), ('hydro', (16, 16, 16), 'RK2trap'),
               ('boussinesq', (32, 48), 'RK2mid'), ('boussinesq', (16, 8, 16), 'RK2trap'),
               ('mhd', (16, 16, 16), 'RK2mid'), ('mhd', (32, 32), 'RK2trap'),
-              ('boussinesq', (64, 64, 64), 'RK2trap'), ('mhd', (64, 64, 64), 'RK2mid')]
+              ('boussinesq', (64, 64, 64), 'RK2trap'), ('mhd', (64, 64, 64), 'RK2mid'),
+              ('mhd', (16, 32, 16), 'RK2trap')]
 
 
 @pytest.mark.parametrize("kind,shape,integ", FUSE_CASES, ids=str)
