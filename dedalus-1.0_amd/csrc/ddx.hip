@@ -252,13 +252,10 @@ enum ColKind { COL_PLAIN, COL_SPECTRAL };
 template <typename T, int N, bool WIDE>
 void col_fast_go(ddx_plan* p, const ColArgs<T>& a, dim3 grid, int dir, bool spectral) {
   const dim3 block(ColPlan<N>::LANES * col_width<N, WIDE>());
-  if (dir > 0) {
-    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true, WIDE>), grid, block, 0, p->stream, a);
-    else hipLaunchKernelGGL((col_fft_fast<T, 1, N, false, WIDE>), grid, block, 0, p->stream, a);
-  } else {
-    if (spectral) hipLaunchKernelGGL((col_fft_fast<T, -1, N, true, WIDE>), grid, block, 0, p->stream, a);
-    else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false, WIDE>), grid, block, 0, p->stream, a);
-  }
+  // spectral prologues belong to backward passes only (launch_col_fast)
+  if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true, WIDE>), grid, block, 0, p->stream, a);
+  else if (dir > 0) hipLaunchKernelGGL((col_fft_fast<T, 1, N, false, WIDE>), grid, block, 0, p->stream, a);
+  else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false, WIDE>), grid, block, 0, p->stream, a);
 }
 
 // Size-specialised column kernel for n in {64, 128, 256, 512}
@@ -294,6 +291,7 @@ void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bo
 
 template <typename T>
 bool launch_col_fast(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
+  if (spectral && dir < 0) return false;   // not a pass of this library: the generic kernel
   switch (a.n) {
     case 64: col_fast_n<T, 64>(p, a, nslabs, nfields, dir, spectral); return true;
     case 128: col_fast_n<T, 128>(p, a, nslabs, nfields, dir, spectral); return true;
@@ -635,22 +633,14 @@ int phys_layout(int physics, int nd, PhysLayout* L) {
   return 0;
 }
 
-// The backward z pass of a 3D right-hand side splits the derived fields:
-// a derivative along z must be taken before the z transform, but one along x
-// or y can wait for the y pass.
-// DDX_ZSPLIT_MODE 1: each distinct (source, with or without d/dz) of the
-// recipes becomes one z-pass output, and the y pass forms every derived field
-// from those with the remaining x and y derivatives (MHD: u, dz ux, dz uy
-// and the same for B; every curl is a two-term y-pass field).
-// DDX_ZSPLIT_MODE 2: fields with a d/dz term, and plain copies, are formed in
-// the z pass; fields without one wait for the y pass and are formed there
-// from plain z-pass outputs (MHD: u, wx, wy and the same for B; wz and Jz are
-// the only two-term y-pass fields).
-// Either way MHD transforms 10 arrays along z instead of 12, hydro 5 instead
-// of 6.  Returns the number of z-pass outputs.
-#ifndef DDX_ZSPLIT_MODE
-#define DDX_ZSPLIT_MODE 2
-#endif
+// The backward z pass of a 3D right-hand side splits the derived fields: a
+// derivative along z must be taken before the z transform, but one along x or
+// y can wait for the y pass.  Fields with a d/dz term, and plain copies, are
+// formed in the z pass; the others wait for the y pass and are formed there
+// from plain z-pass outputs.  MHD: the z pass forms u, wx, wy, B, Jx, Jy (10
+// arrays instead of 12), the y pass copies those and forms wz and Jz; hydro
+// transforms 5 arrays instead of 6, Boussinesq 7 instead of 10.  Returns the
+// number of z-pass outputs.
 int split_z(const PhysLayout& L, Recipe* zrec, Recipe* yrec) {
   int nz = 0;
   // index of the z-pass output that is source s, plain (added if missing)
@@ -661,35 +651,20 @@ int split_z(const PhysLayout& L, Recipe* zrec, Recipe* yrec) {
     if (j == nz) zrec[nz++] = one(s);
     return j;
   };
-  if (DDX_ZSPLIT_MODE == 2) {
-    int zi[DDX_MAXF];
-    for (int q = 0; q < L.nq; ++q) {
-      const Recipe& r = L.rec[q];
-      bool dz = false;
-      for (int t = 0; t < r.nterm; ++t) dz = dz || r.t[t].dim == 2;
-      const bool copy = r.nterm == 1 && r.t[0].dim < 0 && r.t[0].coef == 1.0;
-      zi[q] = -1;
-      if (copy) zi[q] = plain(r.t[0].src);
-      else if (dz) { zi[q] = nz; zrec[nz++] = r; }
-    }
-    for (int q = 0; q < L.nq; ++q) {
-      if (zi[q] >= 0) { yrec[q] = one(zi[q]); continue; }
-      yrec[q] = L.rec[q];
-      for (int t = 0; t < L.rec[q].nterm; ++t) yrec[q].t[t].src = plain(L.rec[q].t[t].src);
-    }
-    return nz;
+  int zi[DDX_MAXF];
+  for (int q = 0; q < L.nq; ++q) {
+    const Recipe& r = L.rec[q];
+    bool dz = false;
+    for (int t = 0; t < r.nterm; ++t) dz = dz || r.t[t].dim == 2;
+    const bool copy = r.nterm == 1 && r.t[0].dim < 0 && r.t[0].coef == 1.0;
+    zi[q] = -1;
+    if (copy) zi[q] = plain(r.t[0].src);
+    else if (dz) { zi[q] = nz; zrec[nz++] = r; }
   }
   for (int q = 0; q < L.nq; ++q) {
+    if (zi[q] >= 0) { yrec[q] = one(zi[q]); continue; }
     yrec[q] = L.rec[q];
-    for (int t = 0; t < L.rec[q].nterm; ++t) {
-      const Term tm = L.rec[q].t[t];
-      const int zd = tm.dim == 2 ? 2 : -1;
-      int j = 0;
-      while (j < nz && !(zrec[j].t[0].src == tm.src && zrec[j].t[0].dim == zd)) ++j;
-      if (j == nz) zrec[nz++] = one(tm.src, zd);
-      yrec[q].t[t].src = j;
-      yrec[q].t[t].dim = tm.dim == 2 ? -1 : tm.dim;
-    }
+    for (int t = 0; t < L.rec[q].nterm; ++t) yrec[q].t[t].src = plain(L.rec[q].t[t].src);
   }
   return nz;
 }

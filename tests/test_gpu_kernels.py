@@ -402,8 +402,9 @@ ZSPLIT_CASES = [('mhd', (64, 64, 64), {}), ('hydro', (32, 256, 64), {}),
 @pytest.mark.parametrize("kind,shape,extra", ZSPLIT_CASES, ids=str)
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_z_split_matches_unsplit(kind, shape, extra, precision):
-    """FP64 3D right-hand sides transform along z only u, dz ux, dz uy (and
-    B, T, dz T) and form the curls and gradients in the y pass.  The states
+    """FP64 3D right-hand sides form the fields with a d/dz term (wx, wy, Jx,
+    Jy, dz T) and the plain copies before the z transforms, and the others
+    (wz, Jz, dx T, dy T) in the y pass from plain z-pass outputs.  The states
     agree with the unsplit dataflow (DDX_NO_ZSPLIT=1, every derived field
     formed before the z pass) to rounding, and with the oracle within 1e-12.
     FP32 plans do not split: the two runs are then identical."""
