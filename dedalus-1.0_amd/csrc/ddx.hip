@@ -65,6 +65,9 @@ struct AxisInfo {
   int c = 0;  // largest kept |frequency index|
   int K = 1;  // kept count: 2c+1 on full axes, c+1 on the x axis
   std::vector<double> kfull;  // k for frequency index m in [0, n)
+  // k(m) = ((ms * inv) * 2) * kny with ms the signed index: the factors, for
+  // kernels that form k from the index instead of reading the table
+  double inv = 1.0, kny = 0.0;
 };
 
 int setup_axis(int n, double L, bool half, AxisInfo* ax, std::string* why) {
@@ -73,6 +76,8 @@ int setup_axis(int n, double L, bool half, AxisInfo* ax, std::string* why) {
   const double kny = (M_PI * (double)n) / L;
   const double inv = 1.0 / (double)n;
   const double thr = 2.0 / 3.0 * kny;
+  ax->inv = inv;
+  ax->kny = kny;
   ax->kfull.resize(n);
   const int npos = (n - 1) / 2 + 1;  // numpy fftfreq split
   std::vector<int> keep(n, 0);
@@ -354,6 +359,7 @@ void geom_bz(const ddx_plan* p, ColArgs<T>& a) {
   a.k_row = p->dkz; a.row_dim = 2;
   a.k_slab = p->dky; a.slab_dim = 1;
   a.k_col = p->dkx;
+  a.kr_inv = p->az.inv; a.kr_kny = p->az.kny;
 }
 
 // Backward y pass.  3D: (Ky, Nz, Kxp) -> (Nz, Ny, Kxp); 2D: (Ky, Kxp) -> (Ny, Kxp)
@@ -371,6 +377,7 @@ void geom_by(const ddx_plan* p, ColArgs<T>& a) {
   a.k_row = p->dky; a.row_dim = 1;
   a.k_slab = nullptr; a.slab_dim = -1;
   a.k_col = p->dkx;
+  a.kr_inv = p->ay.inv; a.kr_kny = p->ay.kny;
 }
 
 // Forward y pass.  3D: (Nz, Ny, Kxp) -> (Ky, Nz, Kxp); 2D: (Ny, Kxp) -> (Ky, Kxp)

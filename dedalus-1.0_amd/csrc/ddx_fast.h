@@ -575,14 +575,18 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
 // tile for the lanes past the last column; row 0 for rows left out), and
 // unused values are zeroed by a select afterwards: loads under lane-dependent
 // or per-term branches each end in a wait for memory, which serialises the
-// column's loads.  Wavenumbers: k_x is one load per lane (the column), the
-// slab axis's one per workgroup, and the row axis's one per element, loaded
-// only by the ROWK variant (a term differentiates along the row axis) and
-// shared by both terms.
+// column's loads.  Wavenumbers: k_x is one table load per lane (the column),
+// the slab axis's one per workgroup, and the row axis's one per element,
+// needed only by the ROWK variant (a term differentiates along the row axis)
+// and shared by both terms.  FP64 passes at n = 128 form the row values from
+// the index as the table does, ((ms * (1/n)) * 2) * kny, bit for bit: that
+// keeps them at 167 VGPRs, three wavefronts per SIMD (172 with the loads);
+// elsewhere the loads, issued with the data, measured faster.
 template <typename T, int N, class PL, bool SPECTRAL, int NT, bool ROWK, int W>
 __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>& a, int lane, int f,
                                            long slab, int col, bool live) {
   constexpr int LANES = PL::LANES, E = PL::E, R0 = PL::R[0];
+  constexpr bool KR_IDX = sizeof(T) == 8 && N == 128;
   // lanes past the last column read the same lane's column of the previous
   // tile: distinct addresses in lines that tile's workgroup has just read
   const int colc = live ? col : (col >= 2 * W ? col - W : a.ncols - 1);
@@ -625,7 +629,10 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
       if (SPECTRAL) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) ld[t][e] = srcp[t][off];
-        if (ROWK) kr[e] = a.k_row[rowc];
+        if (ROWK) {
+          if (KR_IDX) kr[e] = ((double)(m <= a.in_c ? m : m - N) * a.kr_inv) * 2.0 * a.kr_kny;
+          else kr[e] = a.k_row[rowc];
+        }
       } else {
         ld[0][e] = a.in[f][off];
       }

@@ -54,6 +54,9 @@ struct ColArgs {
   int xcd_group;         // 1D grid on XCDs: 1 the fields of a (slab, tile), 2 whole slabs
   int group_hint;        // set by the caller: group the fields per XCD (spectral passes)
   int wide;              // set by the caller: double-width tiles (n = 256)
+  // the row axis's wavenumbers from the index, k = ((ms * inv) * 2) * kny
+  // with ms the signed frequency index (some size-specialised kernels)
+  double kr_inv, kr_kny;
   T scale;
 };
 
