@@ -580,7 +580,7 @@ __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrow
 // needed only by the ROWK variant (a term differentiates along the row axis)
 // and shared by both terms.  FP64 passes at n = 128 form the row values from
 // the index as the table does, ((ms * (1/n)) * 2) * kny, bit for bit: that
-// keeps them at 167 VGPRs, three wavefronts per SIMD (172 with the loads);
+// keeps them at 168 VGPRs, three wavefronts per SIMD (172 with the loads);
 // elsewhere the loads, issued with the data, measured faster.
 template <typename T, int N, class PL, bool SPECTRAL, int NT, bool ROWK, int W>
 __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>& a, int lane, int f,
