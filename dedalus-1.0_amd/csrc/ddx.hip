@@ -719,8 +719,8 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   cpx<T>* mixed;  // (Nz, Ny, Kxp) per field
   // FP64 only: at 256^3 MHD the z pass gets 26 % faster (10 outputs
   // instead of 12) and the y pass 1 % faster: 2.5 % per step.  FP32 at
-  // 128^3 measured 6 % slower per step (its y pass is short of load issue,
-  // not bytes).
+  // 128^3 measured 1-2 % slower per step with the split (its passes are
+  // short of issue slots, not of bytes).
   Recipe zrec[DDX_MAXF], yrec[DDX_MAXF];
   const int nzs = p->zfft && p->zsplit && sizeof(T) == 8 ? split_z(L, zrec, yrec) : L.nq;
   if (p->zfft && nzs < L.nq) {
