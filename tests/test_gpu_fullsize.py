@@ -1,0 +1,94 @@
+"""The 256^3 MHD workload at its full size (the bench configuration), checked
+through properties that hold at any size (GPU).
+
+The oracle needs about 18 s per step at this size, so parity itself is
+tested at smaller sizes (test_gpu_kernels.py); here a few fused steps must
+keep the state solenoidal (projection, induction curl), keep the kx = 0 plane
+Hermitian, lose energy at the viscous and resistive rate the spectrum sets,
+and equal separate update launches bit for bit.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import numpy_path as npth
+
+pytestmark = pytest.mark.gpu
+
+
+def hip():
+    from dedalus import _hip
+    _hip.init()
+    return _hip
+
+
+def energy(comps, weight):
+    return float(sum(np.sum(weight * np.abs(a) ** 2) for a in comps))
+
+
+def test_mhd256_full_size_properties():
+    from dedalus.init_cond.seeded import workload
+    hp = hip()
+    w = workload('mhd3d_256')
+    shape, L, prm = w['shape'], w['length'], w['params']
+    print("case: full-size MHD", shape, flush=True)
+    nsteps, dt = 3, w['dt']
+    states = []
+    for separate in (False, True):
+        if separate:
+            os.environ["DDX_NO_FUSED_UPDATE"] = "1"
+        try:
+            plan = hp.Plan(shape, L)
+        finally:
+            os.environ.pop("DDX_NO_FUSED_UPDATE", None)
+        st = []
+        for name in ('u', 'B'):
+            for a in w['fields'][name]:
+                b = plan.alloc_k()
+                plan.upload_k(b, a)
+                st.append(b)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        p = hp.make_params(hp.MHD, nu=prm['nu'], eta=prm['eta'], rho0=prm['rho0'])
+        plan.step(hp.RK2MID, p, st, bufs[0], bufs[1], bufs[2], nsteps, dt)
+        states.append([plan.download_k(b) for b in st])
+        del plan, st, bufs
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
+    out = states[0]
+
+    c = npth.Component(shape, L)
+    k = [c.k['xyz'[i]] for i in range(3)]
+    kmag = np.sqrt(sum(ki ** 2 for ki in k))
+    for name, comps in (('u', out[:3]), ('B', out[3:])):
+        div = sum(1j * k[i] * comps[i] for i in range(3))
+        scale = np.max(kmag * np.sqrt(sum(np.abs(a) ** 2 for a in comps)))
+        assert np.abs(div).max() < 1e-13 * scale, name
+        for a in comps:
+            plane = a[:, :, 0]
+            n0, n1 = plane.shape
+            mirror = np.conj(plane[(-np.arange(n0)) % n0][:, (-np.arange(n1)) % n1])
+            assert np.abs(plane - mirror).max() < 1e-13 * np.abs(plane).max(), name
+
+    # energy |u|^2 / 2 + |B|^2 / (8 pi rho0) over the full spectrum: the
+    # stored half spectrum counts kx > 0 twice
+    weight = np.full(out[0].shape, 2.0)
+    weight[:, :, 0] = 1.0
+    if shape[-1] % 2 == 0:
+        weight[:, :, -1] = 1.0
+    fb = 1.0 / (4.0 * np.pi * prm['rho0'])
+    e0 = 0.5 * energy(w['fields']['u'], weight) + 0.5 * fb * energy(w['fields']['B'], weight)
+    e1 = 0.5 * energy(out[:3], weight) + 0.5 * fb * energy(out[3:], weight)
+    # the nonlinear terms of the truncated system conserve it, so over a
+    # short time t it falls by t (nu sum k^2 |u|^2 + eta fb sum k^2 |B|^2)
+    # to first order; the second-order term (the spectrum's change under the
+    # nonlinear transfer) is 1e-5 of the loss at 16^3 and 32^3 on the oracle.
+    # A wrong viscous or resistive rate, or a term that does not conserve
+    # energy (e.g. a sign of the Lorentz force), moves the loss by far more.
+    k2 = kmag ** 2
+    diss = (prm['nu'] * sum(np.sum(weight * k2 * np.abs(a) ** 2) for a in w['fields']['u'])
+            + prm['eta'] * fb * sum(np.sum(weight * k2 * np.abs(a) ** 2) for a in w['fields']['B']))
+    loss, pred = 1.0 - e1 / e0, nsteps * dt * diss / e0
+    print("energy loss %.6e, first-order prediction %.6e" % (loss, pred), flush=True)
+    assert abs(loss - pred) < 1e-3 * pred
