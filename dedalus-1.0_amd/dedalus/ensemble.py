@@ -122,6 +122,11 @@ class Ensemble:
     def capture(self, concurrent=True):
         """One graph holding one RK2mid step of every group.
 
+        The capture first takes one eager step (it allocates the workspaces,
+        which cannot be allocated inside a graph), so every member advances
+        by one step; seed or load the members afterwards to start the
+        replays from a given state.
+
         concurrent: the groups are independent, so each group other than the
         first is captured on a side stream forked from and joined back to the
         ensemble's stream; the graph then lets the small groups' launches,
