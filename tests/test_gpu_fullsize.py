@@ -1,11 +1,14 @@
-"""The 256^3 MHD workload at its full size (the bench configuration), checked
-through properties that hold at any size (GPU).
+"""Workloads at their full sizes (GPU).
 
-The oracle needs about 18 s per step at this size, so parity itself is
-tested at smaller sizes (test_gpu_kernels.py); here a few fused steps must
-keep the state solenoidal (projection, induction curl), keep the kx = 0 plane
-Hermitian, lose energy at the viscous and resistive rate the spectrum sets,
-and equal separate update launches bit for bit.
+- 256^3 MHD (the bench configuration): the oracle needs about 18 s per step
+  here, so a few fused steps are checked through properties that hold at any
+  size: the state stays solenoidal (projection, induction curl), the kx = 0
+  plane stays Hermitian, energy falls at the viscous and resistive rate the
+  spectrum sets, and the fused step equals separate update launches bit for
+  bit.
+- The 768-member FP32 ensemble: every member's energy loss at the 2D viscous
+  rate; solenoidal members.
+- 128^3 hydro and 512^2 Boussinesq: a few steps against the oracle itself.
 """
 
 import os
@@ -124,3 +127,38 @@ def test_ensemble768_full_size_properties():
     v = g.plan.download_k(g.state[1])
     div = np.abs(kx * u + ky * v).max()
     assert div < 1e-5 * np.max(np.sqrt(kx ** 2 + ky ** 2) * np.abs(u))
+
+
+@pytest.mark.parametrize("name,nsteps", [('hydro3d_128', 2), ('boussinesq2d_512', 10)], ids=str)
+def test_full_size_parity(name, nsteps):
+    """The 128^3 hydro and 512^2 Boussinesq workloads at their full sizes
+    (FP64) against the oracle, which finishes a few of their steps in
+    seconds: relative L2 within 1e-12 per component."""
+    from dedalus.init_cond.seeded import workload
+    hp = hip()
+    w = workload(name)
+    kind = 'hydro' if name.startswith('hydro') else 'boussinesq'
+    shape, L, prm = w['shape'], w['length'], w['params']
+    print("case: full-size parity", name, shape, nsteps, flush=True)
+    plan = hp.Plan(shape, L)
+    names = ['u'] if kind == 'hydro' else ['u', 'T']
+    st = []
+    for nm in names:
+        for a in w['fields'][nm]:
+            b = plan.alloc_k()
+            plan.upload_k(b, a)
+            st.append(b)
+    bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+    nd = len(shape)
+    bdir = {'x': 0, 'y': 1, 'z': 2}[prm.get('boussinesq_direction', 'z' if nd == 3 else 'y')]
+    p = hp.make_params(hp.HYDRO if kind == 'hydro' else hp.BOUSSINESQ, nu=prm.get('nu', 0.0),
+                       kappa=prm.get('kappa', 0.0), g=prm.get('g', 1.0), alpha_t=prm.get('alpha_t', 1.0),
+                       beta=prm.get('beta', 1.0), bdir=bdir)
+    plan.step(hp.RK2MID, p, st, bufs[0], bufs[1], bufs[2], nsteps, w['dt'])
+    got = [plan.download_k(b) for b in st]
+    ref = npth.run(kind, shape, w['fields'], w['dt'], nsteps, 'RK2mid', L, **prm)
+    want = [a for nm in names for a in ref[nm]]
+    for i, (g, r) in enumerate(zip(got, want)):
+        err = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+        print("  component %d: rel L2 %.2e" % (i, err), flush=True)
+        assert err < 1e-12, (name, i, err)
