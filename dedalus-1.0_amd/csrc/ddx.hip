@@ -149,6 +149,8 @@ struct ddx_plan {
   // derivatives need k_z, and form the other derived fields in the y pass
   // (DDX_NO_ZSPLIT=1 forms every derived field before the z pass)
   bool zsplit = true;
+  // ddx_plan_set_tuning choices, -1: the built-in rule
+  int tune_wide = -1, tune_xcd = -1, tune_yepi = -1;
   int nbatch = 1;
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
@@ -268,7 +270,8 @@ template <typename T, int N>
 void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bool spectral) {
   // double-width tiles at n = 256: FP32, the plain FP64 passes, and the
   // spectral passes whose caller asks for them
-  const bool wide = N == 256 && (sizeof(T) == 4 || !spectral || a.wide);
+  const bool wide = N == 256 && (p->tune_wide >= 0 ? p->tune_wide == 1
+                                                   : (sizeof(T) == 4 || !spectral || a.wide));
   const int W = wide ? col_width<N, true>() : col_width<N, false>();
   const int ntiles = (a.ncols + W - 1) / W;
   dim3 grid(ntiles, nslabs, nfields);
@@ -282,6 +285,7 @@ void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bo
   // less fetch per launch at 256^3); on the z pass that order was 5 % slower.
   a.xcd_group = (spectral && nfields > 1 && (a.row_dim == 2 || a.group_hint) && DDX_COL_XCD_GROUP)
                     ? (a.group_hint ? 2 : 1) : 0;
+  if (spectral && p->tune_xcd >= 0) a.xcd_group = p->tune_xcd;
   if (a.xcd_group == 2) {
     grid = dim3((unsigned)(((nslabs + 7) / 8) * 8 * (long)ntiles * nfields), 1, 1);
   } else if (a.xcd_group) {
@@ -764,7 +768,8 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   // forward column passes (single 2D grids with 64-point y axes: the y pass
   // runs inside the epilogue; batched plans keep the two launches, which
   // measured as fast for 256 members)
-  const bool yepi = !p->zfft && nd == 2 && p->nbatch == 1 && p->fuse_yepi && !p->generic_only &&
+  const bool yepi = !p->zfft && nd == 2 && (p->tune_yepi >= 0 ? p->tune_yepi == 1 : p->nbatch == 1) &&
+                    p->fuse_yepi && !p->generic_only &&
                     p->ay.n == 64 && p->Ky <= 2 * ((64 + 2) / 3) + 1 && L.np <= 3;
   cpx<T>* Uc;
   col_defaults(c);
@@ -1221,6 +1226,22 @@ int ddx_plan_set_graph(ddx_plan* p, int on) {
     p->step_exec = nullptr;
   }
   p->use_graph = on != 0;
+  return 0;
+}
+
+int ddx_plan_set_tuning(ddx_plan* p, int knob, int value) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  const int top = knob == DDX_TUNE_XCD_ORDER ? 2 : 1;
+  if (knob < DDX_TUNE_WIDE_TILES || knob > DDX_TUNE_FUSED_YEPI || value < -1 || value > top)
+    return fail(DDX_ERR_ARG, "unknown tuning knob or value");
+  if (p->step_exec) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipGraphExecDestroy(p->step_exec));
+    p->step_exec = nullptr;
+  }
+  if (knob == DDX_TUNE_WIDE_TILES) p->tune_wide = value;
+  else if (knob == DDX_TUNE_XCD_ORDER) p->tune_xcd = value;
+  else p->tune_yepi = value;
   return 0;
 }
 

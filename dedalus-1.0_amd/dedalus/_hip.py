@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libddx.so")
 F64, F32 = 0, 1
 HYDRO, BOUSSINESQ, MHD = 0, 1, 2
 RK2MID, RK2TRAP = 0, 1
+TUNE_WIDE_TILES, TUNE_XCD_ORDER, TUNE_FUSED_YEPI = 0, 1, 2
 EULER, ETD1, ETD2RK1, ETD2RK2 = 0, 1, 2, 3
 
 ERR_NAMES = {-1: "DDX_ERR_ARG", -2: "DDX_ERR_HIP", -3: "DDX_ERR_UNSUPPORTED",
@@ -71,6 +72,7 @@ SIGNATURES = {
     "ddx_plan_ktable": (_I, [_VP, _I, _DP, ctypes.c_int64]),
     "ddx_plan_set_stream": (_I, [_VP, _VP]),
     "ddx_plan_set_graph": (_I, [_VP, _I]),
+    "ddx_plan_set_tuning": (_I, [_VP, _I, _I]),
     "ddx_alloc": (_I, [_SZ, _PP]),
     "ddx_free": (_I, [_VP]),
     "ddx_h2d": (_I, [_VP, _VP, _VP, _SZ]),
@@ -268,6 +270,11 @@ class Plan:
 
     def set_graph(self, on):
         _check(_lib.ddx_plan_set_graph(self.handle, 1 if on else 0))
+
+    def set_tuning(self, knob, value):
+        """A kernel choice (TUNE_*) that leaves results bit-identical (FP64
+        tile widths: equal to rounding); -1 restores the built-in rule."""
+        _check(_lib.ddx_plan_set_tuning(self.handle, int(knob), int(value)))
 
     def set_stream(self, stream):
         """Launch on an existing hipStream_t (an int handle); None or 0 restores

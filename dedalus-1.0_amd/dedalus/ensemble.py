@@ -163,6 +163,52 @@ class Ensemble:
         self._side = side                            # kept alive with the graph
         return graph
 
+    def autotune(self, steps=4, reps=2):
+        """Choose, per group, the kernel variants that run its step fastest:
+        16-column tiles (256-point groups), the XCD order of the spectral y
+        pass, and the fused forward y pass and epilogue (64-point groups).
+        Every variant gives bit-identical results in FP32 (FP64 tile widths
+        agree to rounding).  Each candidate is timed
+        with HIP events over `steps` eager steps, best of `reps`, and the
+        members' state is restored afterwards.  Returns {shape: choice};
+        capture() afterwards to replay the tuned kernels."""
+        import itertools
+        import torch
+        self.graph = None
+        chosen = {}
+        with torch.cuda.stream(self.stream):
+            for g in self.groups:
+                g.plan.set_graph(False)
+                saved = [g.plan.alloc_k(zero=False) for _ in g.state]
+                for b, s in zip(saved, g.state):
+                    g.plan.copy(b, s)
+                wide = (0, 1) if g.shape[0] == 256 else (-1,)
+                yepi = (0, 1) if g.shape[0] == 64 else (-1,)
+                best = None
+                for cand in itertools.product(wide, (0, 1, 2), yepi):
+                    for knob, v in zip((_hip.TUNE_WIDE_TILES, _hip.TUNE_XCD_ORDER, _hip.TUNE_FUSED_YEPI), cand):
+                        g.plan.set_tuning(knob, v)
+                    g.step(self.params, 1, self.dt)          # first use of the variant
+                    t = None
+                    for _ in range(reps):
+                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        e0.record(self.stream)
+                        g.step(self.params, steps, self.dt)
+                        e1.record(self.stream)
+                        e1.synchronize()
+                        ms = e0.elapsed_time(e1) / steps
+                        t = ms if t is None else min(t, ms)
+                    if best is None or t < best[0]:
+                        best = (t, cand)
+                for knob, v in zip((_hip.TUNE_WIDE_TILES, _hip.TUNE_XCD_ORDER, _hip.TUNE_FUSED_YEPI), best[1]):
+                    g.plan.set_tuning(knob, v)
+                for b, s in zip(saved, g.state):
+                    g.plan.copy(s, b)
+                self.stream.synchronize()
+                chosen[g.shape] = dict(wide_tiles=best[1][0], xcd_order=best[1][1], fused_yepi=best[1][2],
+                                       ms_per_step=best[0])
+        return chosen
+
     def advance(self, nsteps):
         import torch
         with torch.cuda.stream(self.stream):

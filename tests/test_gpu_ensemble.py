@@ -188,3 +188,68 @@ def test_member_reductions(precision):
             w = 1j * k['x'] * v - 1j * k['y'] * u
             assert abs(z[m] - 0.5 * mean_square([w], g.shape)) <= tol * z[m]
             m += 1
+
+
+def test_autotune_keeps_results():
+    """Ensemble.autotune picks kernel variants per group (tile width, XCD
+    order, fused forward y + epilogue); it restores the members' state, and
+    the tuned graph's steps equal the untuned ones bit for bit."""
+    hip()
+    from dedalus.ensemble import Ensemble
+    case("ensemble autotune")
+    groups = (((64, 64), 3), ((256, 256), 2))
+    runs = []
+    for tune in (False, True):
+        ens = Ensemble(groups, nu=5e-4, dt=1e-3, precision="f32")
+        ens.seed_members(2000)
+        if tune:
+            before = [g.read() for g in ens.groups]
+            chosen = ens.autotune(steps=2, reps=1)
+            print("chosen", chosen, flush=True)
+            assert set(chosen) == {(64, 64), (256, 256)}
+            for gb, g in zip(before, ens.groups):
+                for (ua, va), (ub, vb) in zip(gb, g.read()):
+                    assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+        ens.capture()
+        ens.seed_members(2000)
+        ens.advance(4)
+        ens.sync()
+        runs.append([g.read() for g in ens.groups])
+    for ga, gb in zip(*runs):
+        for (ua, va), (ub, vb) in zip(ga, gb):
+            assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+
+
+def test_tuning_knobs_keep_results():
+    """ddx_plan_set_tuning choices: the XCD order and the fused forward y +
+    epilogue give bit-identical steps (batched 64^2 and 256^2 plans, FP64 and
+    FP32); the tile width does in FP32, while in FP64 its 16-column tiles
+    exchange through a buffer of reals, compiled to other instruction
+    sequences: results agree to rounding."""
+    hp = hip()
+    case("tuning knobs")
+    cases = (((64, 64), hp.TUNE_FUSED_YEPI, (0, 1)), ((64, 64), hp.TUNE_XCD_ORDER, (0, 1, 2)),
+             ((256, 256), hp.TUNE_XCD_ORDER, (0, 1, 2)), ((256, 256), hp.TUNE_WIDE_TILES, (0, 1)))
+    nb = 3
+    for prec in (hp.F64, hp.F32):
+        for shape, knob, values in cases:
+            members = [kfields('hydro', shape, 700 + m, L2) for m in range(nb)]
+            outs = []
+            for value in (-1,) + values:
+                bp = hp.Plan(shape, L2, prec, batch=nb)
+                bp.set_tuning(knob, value)
+                st = [bp.alloc_k() for _ in range(2)]
+                for c in range(2):
+                    bp.upload_k(st[c], np.stack([members[m]['u'][c] for m in range(nb)]))
+                bufs = [[bp.alloc_k() for _ in range(2)] for _ in range(3)]
+                bp.step(hp.RK2MID, hp.make_params(hp.HYDRO, nu=1e-3), st, bufs[0], bufs[1], bufs[2], 3, 1e-3)
+                outs.append([bp.download_k(b) for b in st])
+            exact = not (knob == hp.TUNE_WIDE_TILES and prec == hp.F64)
+            for o in outs[1:]:
+                for a, b in zip(outs[0], o):
+                    if exact:
+                        assert np.array_equal(a, b), (shape, knob, prec)
+                    else:
+                        assert rel_l2(b, a) < 1e-14, (shape, knob, prec)
+    with pytest.raises(hp.DDXError):
+        hp.Plan((64, 64), L2).set_tuning(hp.TUNE_XCD_ORDER, 3)
