@@ -18,6 +18,9 @@
 #ifndef DDX_COL_XCD_GROUP
 #define DDX_COL_XCD_GROUP 1
 #endif
+#ifndef DDX_COL_NT
+#define DDX_COL_NT 1
+#endif
 #ifndef DDX_EPI_PRELOAD
 #define DDX_EPI_PRELOAD 1
 #endif
@@ -727,9 +730,18 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   // short of issue slots, not of bytes).
   Recipe zrec[DDX_MAXF], yrec[DDX_MAXF];
   const int nzs = p->zfft && p->zsplit && sizeof(T) == 8 ? split_z(L, zrec, yrec) : L.nq;
+  // 3D grids whose arrays dwarf the caches (>= 2^22 complex per field): the
+  // column passes store nontemporally.  Each such pass gets slower (256^3
+  // MHD z pass 6.7 -> 7.3, forward y 6.8 -> 7.3 ms per 40 launches), and the
+  // kernel reading its output faster (y pass 14.1 -> 13.3, x rows 13.9 ->
+  // 13.3, forward z 4.7 -> 4.4, epilogue 8.1 -> 7.5): 2 % per step.  Only
+  // the y pass and the forward z pass storing so was no faster, and at
+  // 128^3, where the intermediates fit the last-level cache, it was slower.
+  const bool big_nt = DDX_COL_NT && p->zfft && (long)p->az.n * p->ay.n * p->Kxp >= (1L << 22);
   if (p->zfft && nzs < L.nq) {
     geom_bz(p, c);
     for (int j = 0; j < nzs; ++j) { c.rec[j] = zrec[j]; c.out[j] = A + j * fs; }
+    c.nt = big_nt;
     if ((rc = launch_col(p, c, p->Ky, nzs, 1, true, PC_COL_BWD_Z_SPECTRAL, L.nstate))) return rc;
     col_defaults(c);
     geom_by(p, c);
@@ -737,14 +749,17 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
     for (int q = 0; q < L.nq; ++q) { c.rec[q] = yrec[q]; c.out[q] = B + q * fs; }
     c.group_hint = DDX_YSPEC_GROUP;
     c.wide = DDX_YSPEC_WIDE;
+    c.nt = big_nt;
     if ((rc = launch_col(p, c, p->az.n, L.nq, 1, true, PC_COL_BWD_Y_SPECTRAL, nzs))) return rc;
   } else if (p->zfft) {
     geom_bz(p, c);
     for (int q = 0; q < L.nq; ++q) c.out[q] = A + q * fs;
+    c.nt = big_nt;
     if ((rc = launch_col(p, c, p->Ky, L.nq, 1, true, PC_COL_BWD_Z_SPECTRAL, L.nstate))) return rc;
     col_defaults(c);
     geom_by(p, c);
     for (int q = 0; q < L.nq; ++q) { c.in[q] = A + q * fs; c.out[q] = B + q * fs; }
+    c.nt = big_nt;
     if ((rc = launch_col(p, c, p->az.n, L.nq, 1, false, PC_COL_BWD_Y))) return rc;
   } else {
     geom_by(p, c);
@@ -777,10 +792,12 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int o = 0; o < L.np; ++o) c.in[o] = A + o * fs;
   if (p->zfft) {
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
+    c.nt = big_nt;
     if ((rc = launch_col(p, c, p->az.n, L.np, -1, false, PC_COL_FWD_Y))) return rc;
     col_defaults(c);
     geom_fz(p, c);
     for (int o = 0; o < L.np; ++o) { c.in[o] = B + o * fs; c.out[o] = A + o * fs; }
+    c.nt = big_nt;
     if ((rc = launch_col(p, c, p->Ky, L.np, -1, false, PC_COL_FWD_Z))) return rc;
     Uc = A;
   } else if (yepi) {

@@ -654,6 +654,21 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
   }
 }
 
+// Output store of a column pass; nt: as a nontemporal (streaming) store
+// (uniform per launch).
+template <typename T>
+__device__ __forceinline__ void col_store(cpx<T>* p, cpx<T> v, bool nt) {
+  if (nt) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 template <typename T, int D, int N, bool SPECTRAL, bool WIDE>
 __global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<N, WIDE>()))
 col_fft_fast(const ColArgs<T> a) {
@@ -736,7 +751,7 @@ col_fft_fast(const ColArgs<T> a) {
       else if (m <= a.out_c) j = m;
       else if (m >= N - a.out_c) j = m - (N - a.out_rows);
       else j = -1;
-      if (j >= 0) a.out[f][slab * a.out_ss + (long)j * a.out_rs + col] = scale(v[qq * RL + rr], a.scale);
+      if (j >= 0) col_store<T>(&a.out[f][slab * a.out_ss + (long)j * a.out_rs + col], scale(v[qq * RL + rr], a.scale), a.nt);
     }
 }
 

@@ -57,6 +57,7 @@ struct ColArgs {
   // the row axis's wavenumbers from the index, k = ((ms * inv) * 2) * kny
   // with ms the signed frequency index (some size-specialised kernels)
   double kr_inv, kr_kny;
+  int nt;                // set by the caller: nontemporal output stores
   T scale;
 };
 
