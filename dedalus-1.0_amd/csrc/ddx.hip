@@ -21,6 +21,12 @@
 #ifndef DDX_COL_NT
 #define DDX_COL_NT 1
 #endif
+#ifndef DDX_ROW_NT
+#define DDX_ROW_NT 1
+#endif
+#ifndef DDX_EPI_NT
+#define DDX_EPI_NT 1
+#endif
 #ifndef DDX_EPI_PRELOAD
 #define DDX_EPI_PRELOAD 1
 #endif
@@ -731,12 +737,14 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   Recipe zrec[DDX_MAXF], yrec[DDX_MAXF];
   const int nzs = p->zfft && p->zsplit && sizeof(T) == 8 ? split_z(L, zrec, yrec) : L.nq;
   // 3D grids whose arrays dwarf the caches (>= 2^22 complex per field): the
-  // column passes store nontemporally.  Each such pass gets slower (256^3
-  // MHD z pass 6.7 -> 7.3, forward y 6.8 -> 7.3 ms per 40 launches), and the
-  // kernel reading its output faster (y pass 14.1 -> 13.3, x rows 13.9 ->
-  // 13.3, forward z 4.7 -> 4.4, epilogue 8.1 -> 7.5): 2 % per step.  Only
-  // the y pass and the forward z pass storing so was no faster, and at
-  // 128^3, where the intermediates fit the last-level cache, it was slower.
+  // column passes, the x rows and the epilogue store nontemporally.  A pass
+  // that stores so gets slower, and the kernel reading its output faster
+  // (256^3 MHD per 40 launches: z pass 6.7 -> 7.3, forward y 6.8 -> 7.3 with
+  // column stores alone; y pass 14.1 -> 13.3, x rows 13.9 -> 13.3, forward z
+  // 4.7 -> 4.4, epilogue 8.1 -> 7.5): 2 % per step for the column passes,
+  // 1 % more with the x rows and the epilogue.  Only the y pass and the
+  // forward z pass storing so was no faster, and at 128^3, where the
+  // intermediates fit the last-level cache, it was slower.
   const bool big_nt = DDX_COL_NT && p->zfft && (long)p->az.n * p->ay.n * p->Kxp >= (1L << 22);
   if (p->zfft && nzs < L.nq) {
     geom_bz(p, c);
@@ -778,6 +786,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   r.fpr_inv = T(1.0 / (4.0 * M_PI * (prm->physics == DDX_MHD ? prm->rho0 : 1.0)));
   for (int q = 0; q < L.nq; ++q) r.in[q] = mixed + q * fs;
   for (int o = 0; o < L.np; ++o) r.out[o] = A + o * fs;
+  r.nt = big_nt && DDX_ROW_NT;
   if ((rc = launch_row(p, r))) return rc;
 
   // forward column passes (single 2D grids with 64-point y axes: the y pass
@@ -826,6 +835,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   e.galpha = prm->g * prm->alpha_t;
   e.beta = prm->beta;
   e.upd_kind = -1;
+  e.nt = big_nt && DDX_EPI_NT;
   int upd_arrays = 0;
   if (fu) {
     e.upd_kind = fu->kind;

@@ -440,8 +440,8 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
           ra = mk(T(0.5) * s.x, T(0.5) * s.y);
           rb = mk(T(0.5) * d.y, T(-0.5) * d.x);
         }
-        oa[k] = ra;
-        if (hasb) ob[k] = rb;
+        st_nt(&oa[k], ra, a.nt);
+        if (hasb) st_nt(&ob[k], rb, a.nt);
       }
     }
     __syncthreads();
@@ -658,15 +658,7 @@ __device__ __forceinline__ void col_gather(cpx<T> (&v)[PL::E], const ColArgs<T>&
 // (uniform per launch).
 template <typename T>
 __device__ __forceinline__ void col_store(cpx<T>* p, cpx<T> v, bool nt) {
-  if (nt) {
-    typedef T v2 __attribute__((ext_vector_type(2)));
-    v2 w;
-    w.x = v.x;
-    w.y = v.y;
-    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
-  } else {
-    *p = v;
-  }
+  st_nt(p, v, nt);
 }
 
 template <typename T, int D, int N, bool SPECTRAL, bool WIDE>

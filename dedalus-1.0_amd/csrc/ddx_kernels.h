@@ -137,6 +137,7 @@ struct RowArgs {
   int nx, kx, kxp;
   int mode, physics, ndim;
   T fpr_inv;  // 1 / (4 pi rho0)
+  int nt;     // nontemporal output stores (x-row products)
 };
 
 // x-space products.  Inputs per physics (ndim 3 / 2):
@@ -297,7 +298,22 @@ struct EpiArgs {
   const cpx<T>* upd_f1[DDX_MAXF];
   cpx<T>* upd_out[DDX_MAXF];
   const cpx<T>* tw_y;   // twiddles of length Ny (fused forward-y epilogue)
+  int nt;               // nontemporal stores of deriv and upd_out
 };
+
+// store, nontemporal (streaming) when nt is set (uniform per launch)
+template <typename T>
+__device__ __forceinline__ void st_nt(cpx<T>* p, cpx<T> v, bool nt) {
+  if (nt) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
 
 template <typename T>
 __device__ inline void kvec(const KGeom& g, long idx, int& col, T& kx, T& ky, T& kz) {
@@ -450,7 +466,7 @@ __device__ __forceinline__ void epi_etd(const EpiArgs<T>& a, long idx, double k2
     // yv / fv: the mode's y and F1, read by the caller before any write
     const cpx<double> y = c64(yv ? yv[v] : a.upd_y[v][idx]);
     const cpx<double> f1 = (a.upd_kind >= UPD_ETD2RK1) ? c64(fv ? fv[v] : a.upd_f1[v][idx]) : dv;
-    a.upd_out[v][idx] = cT<T>(etd_combine(a.upd_kind, y, f1, dv, Z, p0, p1, p2, dt));
+    st_nt(&a.upd_out[v][idx], cT<T>(etd_combine(a.upd_kind, y, f1, dv, Z, p0, p1, p2, dt)), a.nt);
   }
 }
 
@@ -518,7 +534,7 @@ __device__ __forceinline__ void epi_mode(const EpiArgs<T>& a, long idx, double k
   if (wd) {
 #pragma unroll
     for (int v = 0; v < 6; ++v)
-      if (v < nvar) a.deriv[v][idx] = cT<T>(D[v]);
+      if (v < nvar) st_nt(&a.deriv[v][idx], cT<T>(D[v]), a.nt);
   }
   if (upd) epi_etd(a, idx, k2true, 0, nvar, D, yv, fv);
 }
