@@ -139,6 +139,12 @@ struct ddx_plan {
   void* wsA = nullptr;
   void* wsB = nullptr;
   size_t ws_bytes = 0;
+  // bumped whenever the workspace is reallocated (part of the step graph's key)
+  unsigned long ws_gen = 0;
+  // second-stage derivative of ddx_step when the caller passes no deriv2 and
+  // the ETD sweeps run as separate launches (DDX_NO_FUSED_UPDATE=1)
+  void* f2buf = nullptr;
+  size_t f2_bytes = 0;
   // kernel instrumentation (HIP events around each launch when enabled)
   double* partial = nullptr;  // reduction partials (device)
   std::vector<double> hpartial;
@@ -164,6 +170,10 @@ struct ddx_plan {
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
   std::vector<unsigned char> step_key;
+  // key of the last single step that ran eagerly because it missed the
+  // graph's key: a second call with the same key captures a new graph, so a
+  // dt that changes every step (CFL) never pays for a capture
+  std::vector<unsigned char> miss_key;
   bool prof = false;
   struct Rec { int cat; hipEvent_t a, b; double bytes; };
   std::vector<Rec> recs;
@@ -236,6 +246,16 @@ int ensure_ws(ddx_plan* p, int nfields) {
   per = std::max(per, (long)p->nk);
   const size_t need = (size_t)nfields * per * cpx_bytes(p);
   if (need <= p->ws_bytes) return 0;
+  // the captured step graph launches kernels on the old buffers: drain the
+  // stream and drop it before they are freed (a new block may start at the
+  // old address, so the generation counter, not the address, keys the graph)
+  HIPCHK(hipStreamSynchronize(p->stream));
+  if (p->step_exec) {
+    HIPCHK(hipGraphExecDestroy(p->step_exec));
+    p->step_exec = nullptr;
+    p->step_key.clear();
+  }
+  ++p->ws_gen;
   if (p->wsA) HIPCHK(hipFree(p->wsA));
   if (p->wsB) HIPCHK(hipFree(p->wsB));
   p->wsA = p->wsB = nullptr;
@@ -245,6 +265,35 @@ int ensure_ws(ddx_plan* p, int nfields) {
   p->ws_bytes = need;
   return 0;
 }
+
+// ns arrays of nk complex for the second-stage derivative of ddx_step when
+// the caller passes no deriv2 buffers and the sweeps run as separate launches
+int ensure_f2(ddx_plan* p, int ns) {
+  const size_t need = (size_t)ns * p->nk * cpx_bytes(p);
+  if (need <= p->f2_bytes) return 0;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  if (p->step_exec) {
+    HIPCHK(hipGraphExecDestroy(p->step_exec));
+    p->step_exec = nullptr;
+    p->step_key.clear();
+  }
+  ++p->ws_gen;
+  if (p->f2buf) HIPCHK(hipFree(p->f2buf));
+  p->f2buf = nullptr;
+  p->f2_bytes = 0;
+  if (hipMalloc(&p->f2buf, need) != hipSuccess) return fail(DDX_ERR_NOMEM, "deriv2 scratch allocation failed");
+  p->f2_bytes = need;
+  return 0;
+}
+
+// Device scratch of one right-hand side (two buffers of the mixed layouts)
+// and, for ddx_step without deriv2 buffers and unfused sweeps, the second
+// stage's derivative.
+struct Ws {
+  void* A = nullptr;
+  void* B = nullptr;
+  void* F2 = nullptr;
+};
 
 long ws_field_stride(const ddx_plan* p) {
   const long ny = p->ay.n, nz = p->az.n;
@@ -708,21 +757,33 @@ struct FusedUpdate {
   bool write_deriv;
 };
 
+// Workspace fields a right-hand side of this physics needs in each buffer
+int rhs_ws_fields(int physics, int nd) {
+  PhysLayout L;
+  if (phys_layout(physics, nd, &L)) return -1;
+  return std::max(L.nq, L.np);
+}
+
+// The right-hand side on the workspace ws (ws.A, ws.B hold rhs_ws_fields
+// arrays of ws_field_stride each).  deriv may be null when a fused sweep
+// that does not write it (fu->write_deriv false) follows.
 template <typename T>
 int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx<T>* const* deriv,
-             const cpx<T>* const* forcing, const FusedUpdate<T>* fu = nullptr) {
+             const cpx<T>* const* forcing, const FusedUpdate<T>* fu, const Ws& ws) {
   const int nd = p->ndim;
   PhysLayout L;
   if (phys_layout(prm->physics, nd, &L)) return fail(DDX_ERR_ARG, "unknown physics");
   if (prm->physics == DDX_BOUSSINESQ && (prm->bdir < 0 || prm->bdir >= nd))
     return fail(DDX_ERR_ARG, "boussinesq direction outside the grid's dimensions");
+  const bool need_deriv = !fu || fu->write_deriv;
   for (int s = 0; s < L.nstate; ++s)
-    if (!state[s] || !deriv[s]) return fail(DDX_ERR_ARG, "null state or deriv pointer");
-  int rc = ensure_ws(p, std::max(L.nq, L.np));
-  if (rc) return rc;
+    if (!state[s] || (need_deriv && (!deriv || !deriv[s])))
+      return fail(DDX_ERR_ARG, "null state or deriv pointer");
+  if (!ws.A || !ws.B) return fail(DDX_ERR_ARG, "no workspace");
+  int rc;
   const long fs = ws_field_stride(p);
-  cpx<T>* A = (cpx<T>*)p->wsA;
-  cpx<T>* B = (cpx<T>*)p->wsB;
+  cpx<T>* A = (cpx<T>*)ws.A;
+  cpx<T>* B = (cpx<T>*)ws.B;
 
   // backward: spectral prologue in the first column pass
   ColArgs<T> c;
@@ -823,7 +884,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int o = 0; o < L.np; ++o) e.U[o] = Uc + o * fs;
   for (int s = 0; s < L.nstate; ++s) {
     e.state[s] = state[s];
-    e.deriv[s] = deriv[s];
+    e.deriv[s] = deriv ? deriv[s] : nullptr;
     e.forcing[s] = forcing ? forcing[s] : nullptr;
   }
   e.g = kgeom(p);
@@ -919,17 +980,38 @@ void comp_coeffs(const ddx_params* prm, int nd, int nstate, double* coeff) {
 
 template <typename T>
 int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const* y,
-              cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2,
-              const cpx<T>* const* forcing, int nsteps, double dt) {
+              cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2_in,
+              const cpx<T>* const* forcing, int nsteps, double dt, const Ws* fixed_ws = nullptr) {
   const int nd = p->ndim;
   const int ns = nstate_of(prm, nd);
   if (ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
   if (integrator != DDX_RK2MID && integrator != DDX_RK2TRAP)
     return fail(DDX_ERR_ARG, "unknown integrator");
   for (int i = 0; i < ns; ++i) {
-    if (!y[i] || !d1[i] || !d2[i] || (integrator == DDX_RK2MID && !stage[i]))
+    if (!y[i] || !d1[i] || (d2_in && !d2_in[i]) || (integrator == DDX_RK2MID && !stage[i]))
       return fail(DDX_ERR_ARG, "null buffer in step");
   }
+  int rc;
+  Ws ws;
+  if (fixed_ws) {
+    ws = *fixed_ws;
+  } else {
+    if ((rc = ensure_ws(p, rhs_ws_fields(prm->physics, nd)))) return rc;
+    if (!d2_in && !p->fuse_update && (rc = ensure_f2(p, ns))) return rc;
+    ws.A = p->wsA;
+    ws.B = p->wsB;
+    ws.F2 = p->f2buf;
+  }
+  // the second stage's derivative: the caller's deriv2, or (unfused sweeps)
+  // scratch, or not stored at all (fused sweeps)
+  cpx<T>* d2v[DDX_MAXF] = {};
+  for (int i = 0; i < ns; ++i) {
+    if (d2_in) d2v[i] = d2_in[i];
+    else if (!p->fuse_update) d2v[i] = (cpx<T>*)ws.F2 + (long)i * p->nk;
+  }
+  if (!d2_in && !p->fuse_update && !ws.F2) return fail(DDX_ERR_ARG, "no deriv2 scratch");
+  cpx<T>* const* d2 = d2v;
+  const bool keep_d2 = d2_in != nullptr;
   double coeff[DDX_MAXF];
   comp_coeffs(prm, nd, ns, coeff);
   const int vo = prm->viscosity_order;
@@ -937,47 +1019,44 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
     int rc;
     if (p->fuse_update) {
       // the same two RHS + two ETD sweeps, each sweep inside the preceding
-      // epilogue; deriv2 is not written (nothing reads it after the step)
+      // epilogue; deriv2 is written when the caller passed it
       const cpx<T>* const* yc = (const cpx<T>* const*)y;
       const cpx<T>* const* d1c = (const cpx<T>* const*)d1;
       if (integrator == DDX_RK2MID) {
         // time_step.py:285-305
         FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, stage, coeff, vo, dt / 2.0, true};
-        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1))) return rc;
-        FusedUpdate<T> u2{DDX_ETD2RK2, yc, d1c, y, coeff, vo, dt, false};
-        return rhs_impl<T>(p, prm, stage, d2, forcing, &u2);
+        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws))) return rc;
+        FusedUpdate<T> u2{DDX_ETD2RK2, yc, d1c, y, coeff, vo, dt, keep_d2};
+        return rhs_impl<T>(p, prm, stage, keep_d2 ? d2 : nullptr, forcing, &u2, ws);
       }
       // time_step.py:369-389
       FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, y, coeff, vo, dt, true};
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1))) return rc;
-      FusedUpdate<T> u2{DDX_ETD2RK1, yc, d1c, y, coeff, vo, dt, false};
-      return rhs_impl<T>(p, prm, y, d2, forcing, &u2);
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws))) return rc;
+      FusedUpdate<T> u2{DDX_ETD2RK1, yc, d1c, y, coeff, vo, dt, keep_d2};
+      return rhs_impl<T>(p, prm, y, keep_d2 ? d2 : nullptr, forcing, &u2, ws);
     }
     if (integrator == DDX_RK2MID) {
       // time_step.py:285-305
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, stage, d1, nullptr, coeff, vo, dt / 2.0))) return rc;
-      if ((rc = rhs_impl<T>(p, prm, stage, d2, forcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, stage, d2, forcing, nullptr, ws))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK2, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     } else {
       // time_step.py:369-389
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, y, d1, nullptr, coeff, vo, dt))) return rc;
-      if ((rc = rhs_impl<T>(p, prm, y, d2, forcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d2, forcing, nullptr, ws))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK1, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     }
     return 0;
   };
-  int rc;
-  if (!p->use_graph || p->prof || nsteps < 1) {
+  auto eager = [&]() -> int {
     for (int s = 0; s < nsteps; ++s)
       if ((rc = one_step())) return rc;
     return 0;
-  }
+  };
+  if (fixed_ws || !p->use_graph || p->prof || nsteps < 1) return eager();
   // Capture one step as a graph keyed by everything it depends on; replay it.
-  PhysLayout L;
-  phys_layout(prm->physics, nd, &L);
-  if ((rc = ensure_ws(p, std::max(L.nq, L.np)))) return rc;
   std::vector<unsigned char> key;
   auto add = [&key](const void* q, size_t n) {
     const unsigned char* c = (const unsigned char*)q;
@@ -986,20 +1065,33 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
   add(&integrator, sizeof(integrator));
   add(prm, sizeof(*prm));
   add(&dt, sizeof(dt));
+  add(&keep_d2, sizeof(keep_d2));
   add(y, sizeof(void*) * ns);
   add(stage, sizeof(void*) * ns);
   add(d1, sizeof(void*) * ns);
-  add(d2, sizeof(void*) * ns);
+  add(d2v, sizeof(void*) * ns);
   for (int i = 0; i < ns; ++i) {
     const void* f = forcing ? (const void*)forcing[i] : nullptr;
     add(&f, sizeof(f));
   }
-  add(&p->wsA, sizeof(void*));
+  add(&p->ws_gen, sizeof(p->ws_gen));
+  add(&ws.A, sizeof(void*));
+  add(&ws.B, sizeof(void*));
+  add(&ws.F2, sizeof(void*));
   add(&p->stream, sizeof(void*));
   if (!p->step_exec || key != p->step_key) {
+    // A single step whose key was not seen on the previous call runs
+    // eagerly: a capture pays only once the same step repeats.
+    if (nsteps == 1 && key != p->miss_key) {
+      p->miss_key = key;
+      return eager();
+    }
     if (p->step_exec) {
+      // the previous graph may still be running on the stream
+      HIPCHK(hipStreamSynchronize(p->stream));
       HIPCHK(hipGraphExecDestroy(p->step_exec));
       p->step_exec = nullptr;
+      p->step_key.clear();
     }
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
@@ -1018,6 +1110,7 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
     }
     p->step_key = key;
   }
+  p->miss_key.clear();
   for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(p->step_exec, p->stream));
   return 0;
 }
@@ -1466,27 +1559,48 @@ int ddx_update(ddx_plan* p, int kind, const void* y, void* out, const void* F1, 
 int ddx_rhs(ddx_plan* p, const ddx_params* prm, const void* const* state, void* const* deriv,
             const void* const* forcing) {
   if (check_plan(p) || !prm || !state || !deriv) return fail(DDX_ERR_ARG, "null argument");
+  const int nf = rhs_ws_fields(prm->physics, p->ndim);
+  if (nf < 0) return fail(DDX_ERR_ARG, "unknown physics");
+  int rc = ensure_ws(p, nf);
+  if (rc) return rc;
+  Ws ws;
+  ws.A = p->wsA;
+  ws.B = p->wsB;
   if (p->precision == DDX_F64)
     return rhs_impl<double>(p, prm, (const cpx<double>* const*)state, (cpx<double>* const*)deriv,
-                            (const cpx<double>* const*)forcing);
+                            (const cpx<double>* const*)forcing, nullptr, ws);
   return rhs_impl<float>(p, prm, (const cpx<float>* const*)state, (cpx<float>* const*)deriv,
-                         (const cpx<float>* const*)forcing);
+                         (const cpx<float>* const*)forcing, nullptr, ws);
 }
 
-int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
-             void* const* stage, void* const* deriv1, void* const* deriv2,
-             const void* const* forcing, int nsteps, double dt) {
-  if (check_plan(p) || !prm || !state || !deriv1 || !deriv2) return fail(DDX_ERR_ARG, "null argument");
+}  // extern "C"
+
+namespace {
+
+int step_dispatch(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
+                  void* const* stage, void* const* deriv1, void* const* deriv2,
+                  const void* const* forcing, int nsteps, double dt, const Ws* fixed_ws) {
+  if (check_plan(p) || !prm || !state || !deriv1) return fail(DDX_ERR_ARG, "null argument");
   if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
   if (nsteps < 0) return fail(DDX_ERR_ARG, "negative step count");
   void* const* st = stage ? stage : state;
   if (p->precision == DDX_F64)
     return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
                              (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2,
-                             (const cpx<double>* const*)forcing, nsteps, dt);
+                             (const cpx<double>* const*)forcing, nsteps, dt, fixed_ws);
   return step_impl<float>(p, integrator, prm, (cpx<float>* const*)state, (cpx<float>* const*)st,
                           (cpx<float>* const*)deriv1, (cpx<float>* const*)deriv2,
-                          (const cpx<float>* const*)forcing, nsteps, dt);
+                          (const cpx<float>* const*)forcing, nsteps, dt, fixed_ws);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
+             void* const* stage, void* const* deriv1, void* const* deriv2,
+             const void* const* forcing, int nsteps, double dt) {
+  return step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, nsteps, dt, nullptr);
 }
 
 extern "C++" {

@@ -432,14 +432,15 @@ class Plan:
                             ptr_array([int(d) for d in deriv]), fz))
 
     def step(self, integrator, params, state, stage, deriv1, deriv2, nsteps, dt, forcing=None):
+        """ddx_step: deriv2=None leaves the second right-hand side unstored."""
         st = ptr_array([int(s) for s in stage]) if stage is not None else None
+        d2 = ptr_array([int(d) for d in deriv2]) if deriv2 is not None else None
         fz = None
         if forcing is not None:
             fz = ptr_array([int(f) if f is not None else None for f in forcing])
         _check(_lib.ddx_step(self.handle, integrator, ctypes.byref(params),
                              ptr_array([int(s) for s in state]), st,
-                             ptr_array([int(d) for d in deriv1]),
-                             ptr_array([int(d) for d in deriv2]),
+                             ptr_array([int(d) for d in deriv1]), d2,
                              fz, int(nsteps), float(dt)))
 
 

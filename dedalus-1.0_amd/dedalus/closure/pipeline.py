@@ -8,7 +8,10 @@ Configuration (CLOSURE_512_64):
           dedalus/physics/physics.py:524-525, 569-572)
   coarse  64^2 (8 x 8 box filter), same nu, dt = 2e-3, forcing = B(f)
   pairs   8 seeds (one batched fine run), every 80 fine steps (0.02 time
-          units) after a spin-up of 5 time units, 500 per seed
+          units) after a spin-up of 5 time units, 500 per seed: 4000 pairs;
+          the configuration's training set is the 2000 of them taken every
+          160 fine steps (train_subset), the same time span at half the
+          sampling rate
   training 8000 Adam iterations, batch 32
   network (u, v, omega) -> tau (2 components), 3x3 periodic convolutions,
           widths 32, 5 layers, GELU, BF16 MFMA with FP32 accumulation
@@ -39,8 +42,23 @@ CLOSURE_512_64 = dict(
     dt_fine=2.5e-4, dt_ratio=8, forcing=dict(kmin=3.5, kmax=4.5, rms=0.1), forcing_seed=7,
     spinup=5.0, sample_every=80, pairs_per_seed=500,
     train_seeds=(101, 102, 103, 104, 105, 106, 107, 108), train_iters=8000,
-    eval_seed=201, eval_seeds=(201, 202, 203, 204), width=32, depth=5,
+    eval_seed=201, eval_seeds=(201, 202, 203, 204), width=32, depth=5, train_pairs=2000,
 )
+
+
+def train_subset(X, Y, nseeds, npairs):
+    """The first-in-time-order subsample of time-major pairs (pair j * nseeds +
+    s is sample j of seed s) with npairs pairs: every (N / npairs)-th sample
+    of every seed, so the subset spans the same times."""
+    n = X.shape[0]
+    if npairs >= n:
+        return X, Y
+    per = n // nseeds
+    keep = per * nseeds // npairs
+    if keep * npairs != n or per % keep:
+        raise ValueError("%d pairs do not subsample evenly to %d" % (n, npairs))
+    sl = lambda A: A.view(per, nseeds, *A.shape[1:])[::keep].reshape(npairs, *A.shape[1:])
+    return sl(X), sl(Y)
 
 
 def small_config(**over):
@@ -86,11 +104,11 @@ class Pipeline:
         with torch.cuda.stream(self.stream):
             for b, a in zip(st, u0):
                 F.upload_k(b, a)
-        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(3)]
+        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(2)]
         return st, bufs
 
     def fine_advance(self, st, bufs, nsteps):
-        self.cg.fine.step(_hip.RK2MID, self.prm, st, bufs[0], bufs[1], bufs[2], int(nsteps),
+        self.cg.fine.step(_hip.RK2MID, self.prm, st, bufs[0], bufs[1], None, int(nsteps),
                           self.cfg["dt_fine"], forcing=self.f_fine)
 
     def max_speed(self, plan, st):
@@ -173,13 +191,13 @@ class Pipeline:
             for c in range(2):
                 F.upload_k(f[c], np.stack([self._fk_host[c]] * nb))
                 F.upload_k(st[c], np.stack([u[c] for u in u0]))
-            bufs = [[F.alloc_k() for _ in range(2)] for _ in range(3)]
+            bufs = [[F.alloc_k() for _ in range(2)] for _ in range(2)]
             ubar = [C.alloc_k() for _ in range(2)]
             tau = [C.alloc_k() for _ in range(2)]
             xb = [C.alloc_x() for _ in range(3)]
 
             def advance(n):
-                F.step(_hip.RK2MID, self.prm, st, bufs[0], bufs[1], bufs[2], int(n), cfg["dt_fine"],
+                F.step(_hip.RK2MID, self.prm, st, bufs[0], bufs[1], None, int(n), cfg["dt_fine"],
                        forcing=f)
 
             advance(nspin)
@@ -271,7 +289,7 @@ class CoarseRun:
         self.C = C
         self.dt = pipe.cfg["dt_fine"] * pipe.cfg["dt_ratio"]
         self.state = [C.alloc_k() for _ in range(2)]
-        self.bufs = [[C.alloc_k() for _ in range(2)] for _ in range(3)]
+        self.bufs = [[C.alloc_k() for _ in range(2)] for _ in range(2)]
         self.net = net
         self.graph = None
         m = pipe.coarse_shape
@@ -316,7 +334,7 @@ class CoarseRun:
                    add=[int(b) for b in self.fcx])
             C.fwd_many(self.fx, self.fk)
             forcing = self.fk
-        C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], self.bufs[2],
+        C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,
                1, self.dt, forcing=forcing)
 
     def capture(self):
@@ -347,21 +365,34 @@ def spectrum_of(plan, bufs, shape, length):
     return shell_spectrum([plan.download_k(b) for b in bufs], shape, length)[1]
 
 
+def _window_mean(series, times, t0, t1):
+    sel = (times >= t0 - 1e-12) & (times <= t1 + 1e-12)
+    return np.mean(np.asarray(series)[sel], axis=0)
+
+
 def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None, seeds=None,
-             log=None):
+             log=None, nets=None, windows=None):
     """Run the fine truth and coarse models (with and without closure) from the
-    same coarse-grained states; return spectra and errors.
+    same coarse-grained states; return spectra, enstrophies and errors.
 
     seeds: evaluation seeds (default: the configuration's eval_seed), run
-    together as members of batched fine and coarse plans.  Spectra are
-    Hermitian-weighted shell sums (k = 0..kmax of the coarse grid), averaged
-    over the members and over snapshots t >= avg_from (default T/2).  The
-    error is the mean over 1 <= k <= kc (kc = the coarse 2/3 cutoff) of
-    |E_model(k) / E_truth(k) - 1|."""
+    together as members of batched fine and coarse plans.  nets: {name:
+    (net, scales)} of closures to evaluate side by side (default: net and
+    scales as "closure").  Spectra are Hermitian-weighted shell sums (k =
+    0..kmax of the coarse grid); the enstrophy is <omega^2>/2 as
+    volume_average.py:191-200 computes it (Hermitian-weighted sum of
+    |i kx v - i ky u|^2 / 2 over the coarse modes).  For each averaging
+    window (t0, t1) (default one window [avg_from, T], avg_from = T/2):
+      err[name]            mean over 1 <= k <= kc (the coarse 2/3 cutoff) of
+                           |E_model(k) / E_truth(k) - 1|, spectra averaged
+                           over the members and the snapshots in the window;
+      enstrophy_err[name]  |Z_model / Z_truth - 1| of the window means;
+      member_err[name]     the spectrum error of each member alone."""
     cfg = pipe.cfg
     seeds = list(seeds) if seeds else [cfg["eval_seed"]]
     nb = len(seeds)
     avg_from = T / 2 if avg_from is None else avg_from
+    windows = list(windows) if windows else [(avg_from, T)]
     L = cfg["length"]
     cs = pipe.coarse_shape
     ratio = cfg["dt_ratio"]
@@ -371,13 +402,21 @@ def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None,
     nspin = int(round(cfg["spinup"] / cfg["dt_fine"]))
     cg = CoarseGrainer(cfg["fine_shape"], cfg["factor"], L, stream=pipe.stream.cuda_stream, batch=nb)
     F, C = cg.fine, cg.coarse
+    if nets is None:
+        nets = {"closure": (net, scales)} if net is not None else {}
     runs = {"none": pipe.coarse_run(None, batch=nb)}
-    if net is not None:
-        runs["closure"] = pipe.coarse_run(net, scales, batch=nb)
+    for name, (nt, sc) in nets.items():
+        runs[name] = pipe.coarse_run(nt, sc, batch=nb)
 
     def member_spectra(plan, bufs):
         arrs = [plan.download_k(b) for b in bufs]
-        return np.mean([shell_spectrum([a[m] for a in arrs], cs, L)[1] for m in range(nb)], axis=0)
+        return np.array([shell_spectrum([a[m] for a in arrs], cs, L)[1] for m in range(nb)])
+
+    def member_enstrophy(plan, bufs):
+        out = plan.kreduce_members(bufs[0], 1, bufs[1])
+        z = np.empty(nb)
+        plan.d2h(z, out)
+        return 0.5 * z
 
     with torch.cuda.stream(pipe.stream):
         f = [F.alloc_k() for _ in range(2)]
@@ -386,10 +425,10 @@ def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None,
         for c in range(2):
             F.upload_k(f[c], np.stack([pipe._fk_host[c]] * nb))
             F.upload_k(st[c], np.stack([u[c] for u in u0]))
-        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(3)]
+        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(2)]
 
         def advance(n):
-            F.step(_hip.RK2MID, pipe.prm, st, bufs[0], bufs[1], bufs[2], int(n), cfg["dt_fine"],
+            F.step(_hip.RK2MID, pipe.prm, st, bufs[0], bufs[1], None, int(n), cfg["dt_fine"],
                    forcing=f)
 
         ubar = [C.alloc_k() for _ in range(2)]
@@ -400,7 +439,8 @@ def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None,
             r.set_state(ubar)
             r.capture()
             r.set_state(ubar)
-        spec = {k: [] for k in ["truth"] + list(runs)}
+        spec = {k: [] for k in ["truth"] + list(runs)}     # [snap][member][k]
+        zeta = {k: [] for k in ["truth"] + list(runs)}     # [snap][member]
         times = []
         for sn in range(nsnap):
             advance(csteps * ratio)
@@ -410,22 +450,34 @@ def evaluate(pipe, net=None, scales=None, T=10.0, snap_every=0.5, avg_from=None,
                 r.advance(csteps)
             times.append((sn + 1) * csteps * dtc)
             spec["truth"].append(member_spectra(C, ubar))
+            zeta["truth"].append(member_enstrophy(C, ubar))
             for name, r in runs.items():
                 spec[name].append(member_spectra(r.C, r.state))
+                zeta[name].append(member_enstrophy(r.C, r.state))
         C.sync()
     times = np.array(times)
-    sel = times >= avg_from - 1e-12
     kc = C.cut[0]
-    mean = {k: np.mean(np.array(v)[sel], axis=0) for k, v in spec.items()}
     ks = np.arange(1, kc + 1)
-    err = {k: float(np.mean(np.abs(mean[k][ks] / mean["truth"][ks] - 1.0)))
-           for k in runs}
-    ens = {k: [float(np.sum(np.arange(len(e)) ** 2 * e)) for e in v] for k, v in spec.items()}
-    res = dict(times=times.tolist(), err=err, kc=int(kc), seeds=seeds,
-               spectra={k: v.tolist() for k, v in mean.items()},
-               enstrophy=ens, energy={k: [float(e.sum()) for e in v] for k, v in spec.items()})
+    res = dict(times=times.tolist(), kc=int(kc), seeds=seeds, windows=windows,
+               errs=[], enstrophy_errs=[], member_errs=[])
+    for (t0, t1) in windows:
+        mean = {k: _window_mean(v, times, t0, t1).mean(axis=0) for k, v in spec.items()}
+        memb = {k: _window_mean(v, times, t0, t1) for k, v in spec.items()}
+        zm = {k: float(_window_mean(v, times, t0, t1).mean()) for k, v in zeta.items()}
+        res["errs"].append({k: float(np.mean(np.abs(mean[k][ks] / mean["truth"][ks] - 1.0)))
+                            for k in runs})
+        res["enstrophy_errs"].append({k: abs(zm[k] / zm["truth"] - 1.0) for k in runs})
+        res["member_errs"].append({k: [float(np.mean(np.abs(memb[k][m][ks] / memb["truth"][m][ks] - 1.0)))
+                                       for m in range(nb)] for k in runs})
+        if (t0, t1) == windows[0]:
+            res["spectra"] = {k: v.tolist() for k, v in mean.items()}
+    res["err"] = res["errs"][0]
+    res["enstrophy_err"] = res["enstrophy_errs"][0]
+    res["enstrophy"] = {k: np.asarray(v).mean(axis=1).tolist() for k, v in zeta.items()}
+    res["energy"] = {k: [float(np.asarray(e).mean(axis=0).sum()) for e in v] for k, v in spec.items()}
     if log:
-        log("evaluate: spectrum error " + ", ".join("%s %.3f" % kv for kv in err.items()))
+        log("evaluate: spectrum error " + ", ".join("%s %.3f" % kv for kv in res["err"].items())
+            + "; enstrophy error " + ", ".join("%s %.3f" % kv for kv in res["enstrophy_err"].items()))
     return res
 
 
@@ -436,6 +488,36 @@ def time_steps(pipe, run, nsteps=200):
         pipe.stream.synchronize()
         t0 = time.perf_counter()
         run.advance(nsteps)
+        pipe.stream.synchronize()
+        return (time.perf_counter() - t0) / nsteps * 1e3
+
+
+def time_fine_batched(pipe, batch, nsteps=50, seed=None):
+    """Mean wall time per step (ms) of `batch` plain fine runs advanced
+    together on one batched plan (every member the same forced run; one
+    graph per step, replayed)."""
+    cfg = pipe.cfg
+    seed = cfg["eval_seed"] if seed is None else seed
+    if batch == 1:
+        return time_fine_steps(pipe, nsteps, seed)
+    F = _hip.Plan(cfg["fine_shape"], cfg["length"], batch=batch)
+    F.set_stream(pipe.stream.cuda_stream)
+    u0 = initial_velocity(cfg["fine_shape"], cfg["length"], seed)
+    with torch.cuda.stream(pipe.stream):
+        f = [F.alloc_k() for _ in range(2)]
+        st = [F.alloc_k() for _ in range(2)]
+        for c in range(2):
+            F.upload_k(f[c], np.stack([pipe._fk_host[c]] * batch))
+            F.upload_k(st[c], np.stack([u0[c]] * batch))
+        bufs = [[F.alloc_k() for _ in range(2)] for _ in range(2)]
+
+        def adv(n):
+            F.step(_hip.RK2MID, pipe.prm, st, bufs[0], bufs[1], None, int(n), cfg["dt_fine"],
+                   forcing=f)
+        adv(5)
+        pipe.stream.synchronize()
+        t0 = time.perf_counter()
+        adv(nsteps)
         pipe.stream.synchronize()
         return (time.perf_counter() - t0) / nsteps * 1e3
 

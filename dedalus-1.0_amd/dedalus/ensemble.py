@@ -64,7 +64,7 @@ class Group:
         if stream:
             self.plan.set_stream(stream)
         self.state = [self.plan.alloc_k() for _ in range(2)]
-        self.bufs = [[self.plan.alloc_k() for _ in range(2)] for _ in range(3)]
+        self.bufs = [[self.plan.alloc_k() for _ in range(2)] for _ in range(2)]
 
     @property
     def points(self):
@@ -81,7 +81,7 @@ class Group:
         return [(arrs[0][m], arrs[1][m]) for m in range(self.n)]
 
     def step(self, params, nsteps, dt):
-        self.plan.step(_hip.RK2MID, params, self.state, self.bufs[0], self.bufs[1], self.bufs[2],
+        self.plan.step(_hip.RK2MID, params, self.state, self.bufs[0], self.bufs[1], None,
                        int(nsteps), dt)
 
 

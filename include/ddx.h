@@ -111,8 +111,9 @@ int ddx_plan_ktable(const ddx_plan* plan, int axis, double* out, int64_t n);
 int ddx_plan_set_stream(ddx_plan* plan, void* stream);
 /* ddx_step replays one captured HIP graph per step (on = 1, the default
  * unless DDX_NO_GRAPH=1) or launches its kernels one by one (on = 0).  The
- * graph is recaptured whenever the buffers, parameters or dt change.  No
- * reference counterpart: launch overhead matters on small grids. */
+ * graph is recaptured when the buffers, parameters or dt change and the new
+ * step repeats (see ddx_step).  No reference counterpart: launch overhead
+ * matters on small grids. */
 int ddx_plan_set_graph(ddx_plan* plan, int on);
 /* Kernel choices of a plan that do not change its results, for autotuning:
  * every choice is bit-identical, except that FP64 16-column tiles exchange
@@ -223,11 +224,21 @@ int ddx_rhs(ddx_plan* plan, const ddx_params* params, const void* const* state,
             void* const* deriv, const void* const* forcing);
 /* nsteps RK2mid / RK2trap steps of size dt on compact state in place.
  * stage, deriv1, deriv2 are caller buffers with the state's component
- * count (the reference integrator's data2 / deriv1 / deriv2); stage is
- * unused by RK2trap and may be NULL there.  forcing is as in ddx_rhs and
+ * count (the reference integrator's data2 / deriv1 / deriv2,
+ * time_step.py:262-264); stage is unused by RK2trap and may be NULL there.
+ * After the call, as after the reference's last do_advance: deriv1 holds
+ * F(y) of the last step's start, stage (RK2mid) the midpoint state, and
+ * deriv2 the second right-hand side (F(stage) for RK2mid, F of the ETD1
+ * state for RK2trap; time_step.py:297, 381).  deriv2 may be NULL: the
+ * second right-hand side is then not stored, which saves one write of every
+ * component per step.  forcing is as in ddx_rhs and
  * is added at every right-hand side of the nsteps steps: the reference's
  * set_velocity_forcing / set_thermal_forcing hook with a function that does
- * not depend on the state (physics.py:524-525, 569-572). */
+ * not depend on the state (physics.py:524-525, 569-572).
+ * Launches: a call with nsteps > 1, or a single step repeating the previous
+ * call's buffers, parameters and dt, replays one captured HIP graph per
+ * step; any other single step (e.g. dt changing every step under a CFL
+ * condition) launches its kernels directly. */
 int ddx_step(ddx_plan* plan, int integrator, const ddx_params* params, void* const* state,
              void* const* stage, void* const* deriv1, void* const* deriv2,
              const void* const* forcing, int nsteps, double dt);

@@ -321,6 +321,118 @@ def test_graph_replays_match_eager(kind, shape):
             assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("kind,shape", [('hydro', (64, 64)), ('mhd', (16, 16, 16))], ids=str)
+def test_graph_after_workspace_growth(kind, shape):
+    """A captured step, then a multi-field transform that reallocates the
+    plan's workspace, then more steps: equal to eager launches bit for bit
+    (the graph must not replay onto the freed workspace)."""
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("graph after workspace growth", kind, shape)
+    kf = kfields(kind, shape, 97, L)
+    prm = params_struct(hp, kind, nd, {'nu': 1e-3, 'eta': 1e-3})
+    runs = []
+    for graph in (False, True):
+        plan = hp.Plan(shape, L)
+        plan.set_graph(graph)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        snaps = []
+        plan.step(hp.RK2MID, prm, st, bufs[0], bufs[1], bufs[2], 2, 1e-3)
+        snaps.append([plan.download_k(b) for b in st])
+        # 14 fields: more than any right-hand side's workspace
+        xs = [plan.alloc_x() for _ in range(14)]
+        ks = [plan.alloc_k() for _ in range(14)]
+        for i in range(14):
+            plan.copy(ks[i], st[i % len(st)])
+        plan.bwd_many(ks, xs)
+        plan.fwd_many(xs, ks)
+        for _ in range(2):
+            plan.step(hp.RK2MID, prm, st, bufs[0], bufs[1], bufs[2], 2, 1e-3)
+            snaps.append([plan.download_k(b) for b in st])
+        snaps.append([plan.download_k(b) for b in ks])
+        runs.append(snaps)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_single_steps_with_changing_dt():
+    """Single steps whose dt changes every call (a CFL condition) run eagerly;
+    once a step repeats it is captured and replayed.  Every state equals the
+    plan without graphs bit for bit, and the last one the oracle."""
+    hp = hip()
+    shape = (32, 32)
+    L = (2 * np.pi,) * 2
+    case("single steps, changing dt", shape)
+    kf = kfields('hydro', shape, 98, L)
+    prm = params_struct(hp, 'hydro', 2, {'nu': 2e-3})
+    dts = [1e-2, 9e-3, 8.5e-3, 8e-3, 8e-3, 8e-3, 8e-3, 7e-3, 7e-3]
+    runs = []
+    for graph in (False, True):
+        plan = hp.Plan(shape, L)
+        plan.set_graph(graph)
+        st = upload_fields(plan, 'hydro', kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        snaps = []
+        for dt in dts:
+            plan.step(hp.RK2MID, prm, st, bufs[0], bufs[1], bufs[2], 1, dt)
+            snaps.append([plan.download_k(b) for b in st])
+        runs.append(snaps)
+    for a, b in zip(*runs):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    ph = npth.make_physics('hydro', shape, L, nu=2e-3)
+    data = npth.load_state(ph, kf)
+    ti = npth.RK2mid(ph)
+    for dt in dts:
+        ti.do_advance(data, dt)
+    ref = npth.read_state(data)
+    for c in range(2):
+        assert rel_l2(runs[1][-1][c], ref['u'][c]) < 1e-12
+
+
+@pytest.mark.parametrize("kind,shape,integ", [('hydro', (64, 64), 'RK2mid'), ('mhd', (16, 16, 16), 'RK2mid'),
+                                                ('boussinesq', (32, 64), 'RK2trap')], ids=str)
+def test_step_deriv2_optional(kind, shape, integ):
+    """ddx_step without deriv2 buffers gives the same state bit for bit; with
+    them (RK2mid) deriv2 equals ddx_rhs of the midpoint state left in stage,
+    and deriv1 ddx_rhs of the last step's start."""
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("deriv2 optional", kind, shape, integ)
+    kf = kfields(kind, shape, 99, L)
+    p = {'nu': 1e-3, 'eta': 2e-3, 'kappa': 1e-3}
+    prm = params_struct(hp, kind, nd, p)
+    integrator = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+    out = []
+    for keep in (True, False):
+        plan = hp.Plan(shape, L)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        plan.step(integrator, prm, st, bufs[0], bufs[1], bufs[2] if keep else None, 3, 2e-3)
+        y3 = [plan.download_k(b) for b in st]
+        out.append(y3)
+        if keep and integ == 'RK2mid':
+            d2 = [plan.download_k(b) for b in bufs[2]]
+            chk = [plan.alloc_k() for _ in st]
+            plan.rhs(prm, bufs[0], chk)
+            for a, b in zip(d2, [plan.download_k(b) for b in chk]):
+                assert np.array_equal(a, b)
+            # deriv1 = F(y_2): replay two steps from the start
+            st2 = upload_fields(plan, kind, kf)
+            plan.step(integrator, prm, st2, bufs[0], bufs[1], bufs[2], 2, 2e-3)
+            plan.rhs(prm, st2, chk)
+            want = [plan.download_k(b) for b in chk]
+            plan.step(integrator, prm, st2, bufs[0], bufs[1], bufs[2], 1, 2e-3)
+            for a, b in zip([plan.download_k(b) for b in bufs[1]], want):
+                assert np.array_equal(a, b)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("shape,integ", [((32, 32), 'RK2mid'), ((16, 16, 16), 'RK2trap'),
                                          ((64, 48), 'RK2mid')], ids=str)
 def test_steps_with_constant_forcing(shape, integ):
