@@ -378,12 +378,73 @@ class IncompressibleHydro(Physics):
         self.forcing_functions['VelocityForcing'] = func
 
     def RHS(self, data, deriv):
-        """RHS(u) = -P[(u.grad)u] (+ forcing), physics.py:527-586."""
+        """RHS(u) = -P[(u.grad)u] (+ forcing), physics.py:527-586.
+
+        The three built-in classes run the fused device right-hand side.  An
+        object of any other class (a user subclass that inherits or calls
+        this RHS) runs the reference's operator sequence, in which the
+        pressure projection happens only when the object's class is exactly
+        the one whose RHS projects (physics.py:575, 697, 805)."""
         if self._first_rhs:
             self._setup_integrating_factors(deriv)
             self._first_rhs = False
-        Physics.RHS(self, data, deriv, zero_outputs=False)
-        self._fused_rhs(data, deriv)
+        if type(self) in FUSED_CLASSES:
+            Physics.RHS(self, data, deriv, zero_outputs=False)
+            self._fused_rhs(data, deriv)
+        else:
+            self._operator_rhs(data, deriv)
+
+    def _operator_rhs(self, data, deriv):
+        """The reference's RHS bodies operator by operator (physics.py:543-586,
+        676-712, 782-815), each operator a device kernel."""
+        self._check_supported()
+        Physics.RHS(self, data, deriv)
+        ms = self.aux_fields['mathscalar']
+        mv = self.aux_fields['mathvector']
+        plan = _plan(data['u'])
+        forcing = self._forcing(data)
+
+        def scaled(out, terms, accumulate=False):
+            plan.kcombine(out._kdev() if accumulate else out._kout(), terms, accumulate=accumulate)
+            out._mark('kspace')
+
+        # inertial term (physics.py:554-556), forcing (:570-572)
+        self.XgradY(data['u'], data['u'], ms, mv, deriv['u'])
+        for i in self.dims:
+            c = deriv['u'][i]
+            scaled(c, [(c._kdev(), -1.0, -1)])
+            if forcing is not None and forcing[i] is not None:
+                scaled(c, [(forcing[i], 1.0, -1)], accumulate=True)
+        if type(self) is IncompressibleHydro:
+            self.pressure_projection(data, deriv)
+        if isinstance(self, BoussinesqHydro):
+            p = self.parameters
+            d = self._bdir()
+            # buoyancy (:690-694), projection (:697-698)
+            scaled(deriv['u'][d], [(data['T'][0]._kdev(), p['g'] * p['alpha_t'], -1)], accumulate=True)
+            if type(self) is BoussinesqHydro:
+                self.pressure_projection(data, deriv)
+            # temperature advection and stratification (:700-708), forcing (:710-712)
+            T = deriv['T'][0]
+            self.XgradY(data['u'], data['T'], ms, mv, deriv['T'])
+            scaled(T, [(T._kdev(), -1.0, -1), (data['u'][d]._kdev(), -p['beta'], -1)])
+            if forcing is not None and forcing[self.ndim] is not None:
+                scaled(T, [(forcing[self.ndim], 1.0, -1)], accumulate=True)
+        if isinstance(self, IncompressibleMHD):
+            mv2 = self.aux_fields['mathvector2']
+            fpr = 4 * np.pi * self.parameters['rho0']
+            # Lorentz force (:794-802), projection (:804-806)
+            cur = ms if self.ndim == 2 else mv
+            self.curlX(data['B'], cur)
+            self.XcrossY(cur, data['B'], mv2)
+            for i in self.dims:
+                scaled(deriv['u'][i], [(mv2[i]._kdev(), 1.0 / fpr, -1)], accumulate=True)
+            if type(self) is IncompressibleMHD:
+                self.pressure_projection(data, deriv)
+            # induction (:808-815)
+            e = ms if self.ndim == 2 else mv
+            self.XcrossY(data['u'], data['B'], e)
+            self.curlX(e, deriv['B'])
 
     def pressure_projection(self, data, deriv):
         """Solenoidal projection of deriv['u'] (physics.py:588-599)."""
@@ -475,4 +536,4 @@ class IncompressibleMHD(IncompressibleHydro):
         self.dtlist.append(dx / self.max_alfven_speed(data))
 
 
-FUSED_RHS = (IncompressibleHydro.RHS, BoussinesqHydro.RHS, IncompressibleMHD.RHS)
+FUSED_CLASSES = (IncompressibleHydro, BoussinesqHydro, IncompressibleMHD)

@@ -20,7 +20,7 @@ import numpy as np
 
 from dedalus import _hip
 from dedalus.data_objects.representations import IntegratingFactor
-from dedalus.physics.physics import FUSED_RHS
+from dedalus.physics.physics import FUSED_CLASSES
 from dedalus.utils.logger import mylog
 from dedalus.utils.parallelism import com_sys
 from dedalus.utils.timer import timer
@@ -101,10 +101,20 @@ class TimeStepBase:
         params = {}
         for k, v in self.RHS.parameters.items():
             params[k] = v.tolist() if isinstance(v, np.ndarray) else v
+        # what restart() needs to rebuild the objects the reference pickles
+        # (restart.py:40-69): classes, parameters, forcing-function names and
+        # the integrator's clock and stopping controls
+        integ = {'CFL': self.CFL, 'sim_stop_time': self.sim_stop_time,
+                 'wall_stop_time': self.wall_stop_time, 'stop_iteration': self.stop_iteration,
+                 'save_cadence': self.save_cadence, 'max_save_period': self.max_save_period,
+                 'nsnap': self._nsnap + 1, 'tlastsnap': self.time,
+                 'dt_old': float(self.dt_old)}
         meta = {'physics': self.RHS.__class__.__name__, 'shape': list(self.RHS.shape),
                 'length': list(self.RHS.length), 'parameters': params,
+                'forcing_function_names': dict(self.RHS._forcing_function_names),
                 'integrator': self.__class__.__name__, 'time': self.time,
-                'iteration': self.iteration, 'hg_version': hg_version}
+                'iteration': self.iteration, 'integrator_state': integ,
+                'hg_version': hg_version}
         with open(os.path.join(pathname, 'dedalus_obj_%04i.json' % com_sys.myproc), 'w') as f:
             json.dump(meta, f, default=str)
         group = {'time': np.array(self.time)}
@@ -150,7 +160,9 @@ class RKBase(TimeStepBase):
 
     def _fusable(self):
         ph = self.RHS
-        return (getattr(type(ph), 'RHS', None) in FUSED_RHS and not ph.forcing_functions
+        # exactly a built-in class: a subclass may rely on the reference's
+        # class-dependent projection (physics.py:575, 697, 805)
+        return (type(ph) in FUSED_CLASSES and not ph.forcing_functions
                 and ph._phys_id is not None)
 
     def _comps(self, sd):

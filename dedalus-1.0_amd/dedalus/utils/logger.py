@@ -1,4 +1,10 @@
-"""Logging adapter with the process rank (dedalus/utils/logger.py:49-72)."""
+"""The package log, ``mylog`` (the name the reference exports from
+dedalus/utils/logger.py and its samples use).
+
+One handler routes every record by level: debug lines go to stderr from
+every process, info and above to stdout from process 0 only, each line
+tagged with the process rank.  The level comes from ``decfg utils.loglevel``.
+"""
 
 import logging
 import sys
@@ -6,35 +12,32 @@ import sys
 from dedalus.config import decfg
 from dedalus.utils.parallelism import com_sys
 
-
-class SingleLevelFilter(logging.Filter):
-    def __init__(self, passlevel, reject):
-        logging.Filter.__init__(self)
-        self.passlevel = passlevel
-        self.reject = reject
-
-    def filter(self, record):
-        if self.reject:
-            return record.levelno != self.passlevel
-        return record.levelno == self.passlevel
+LINE = "%(asctime)s Dedalus [%(levelname)s] rank %(rank)d: %(message)s"
 
 
-ufstring = "%(asctime)s %(name)-3s: [%(levelname)-9s] %(proc)i %(message)s"
-formatter = logging.Formatter(ufstring)
-extras = {'proc': com_sys.myproc}
+class RankRouter(logging.Handler):
+    """Debug records to stderr on every rank; the rest to stdout on rank 0."""
 
-dedaluslog = logging.getLogger("Dedalus")
-dedaluslog.setLevel(getattr(logging, decfg.get('utils', 'loglevel').upper()))
-mylog = logging.LoggerAdapter(dedaluslog, extras)
+    def __init__(self, rank):
+        logging.Handler.__init__(self, logging.DEBUG)
+        self.rank = rank
+        self.setFormatter(logging.Formatter(LINE))
 
-if not dedaluslog.handlers:
-    debug_handler = logging.StreamHandler(sys.stderr)
-    debug_handler.setLevel(logging.DEBUG)
-    debug_handler.setFormatter(formatter)
-    debug_handler.addFilter(SingleLevelFilter(logging.DEBUG, False))
-    dedaluslog.addHandler(debug_handler)
-    if com_sys.myproc == 0:
-        info_handler = logging.StreamHandler(sys.stdout)
-        info_handler.setLevel(logging.INFO)
-        info_handler.setFormatter(formatter)
-        dedaluslog.addHandler(info_handler)
+    def emit(self, record):
+        if record.levelno > logging.DEBUG and self.rank != 0:
+            return
+        stream = sys.stderr if record.levelno <= logging.DEBUG else sys.stdout
+        try:
+            stream.write(self.format(record) + "\n")
+            stream.flush()
+        except Exception:
+            self.handleError(record)
+
+
+_base = logging.getLogger("Dedalus")
+_base.setLevel(getattr(logging, decfg.get('utils', 'loglevel').upper()))
+if not any(isinstance(h, RankRouter) for h in _base.handlers):
+    _base.addHandler(RankRouter(com_sys.myproc))
+_base.propagate = False
+
+mylog = logging.LoggerAdapter(_base, {'rank': com_sys.myproc})

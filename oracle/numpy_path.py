@@ -399,7 +399,10 @@ class IncompressibleHydro(Physics):
 
     def RHS(self, data, deriv):
         self._hydro_rhs(data, deriv)
-        self.pressure_projection(data, deriv)
+        # the projection runs only for exactly this class (physics.py:575):
+        # a subclass that inherits this RHS gets the unprojected tendency
+        if type(self) is IncompressibleHydro:
+            self.pressure_projection(data, deriv)
 
 
 class BoussinesqHydro(Physics):
@@ -433,7 +436,8 @@ class BoussinesqHydro(Physics):
         c = deriv['u'][d]
         c.require_space('kspace')
         c.kdata += ms['kspace']
-        self.pressure_projection(data, deriv)                           # :697-698
+        if type(self) is BoussinesqHydro:                               # :697-698
+            self.pressure_projection(data, deriv)
         # temperature advection (:700-703)
         self.XgradY(data['u'], data['T'], ms, mv, deriv['T'])
         deriv['T'].comps[0].require_space('kspace')
@@ -481,7 +485,8 @@ class IncompressibleMHD(Physics):
             c = deriv['u'][i]
             c.require_space('kspace')
             c.kdata += mv2[i]['kspace'] / fpr
-        self.pressure_projection(data, deriv)                           # :804-806
+        if type(self) is IncompressibleMHD:                             # :804-806
+            self.pressure_projection(data, deriv)
         # induction (:808-815)
         if self.ndim == 2:
             self.XcrossY(data['u'], data['B'], ms)

@@ -91,8 +91,19 @@ def test_advance_matches_oracle(kind, shape, p, integ):
     ti = (m.RK2mid if integ == 'RK2mid' else m.RK2trap)(RHS)
     for _ in range(6):
         ti.advance(data, 5e-3)
-    ref = npth.run(kind, shape, kf, 5e-3, 6, integ, **dict(p))
-    compare(data, ref)
+    ph = npth.make_physics(kind, shape, **dict(p))
+    odata = npth.load_state(ph, kf)
+    oti = (npth.RK2mid if integ == 'RK2mid' else npth.RK2trap)(ph)
+    for _ in range(6):
+        oti.do_advance(odata, 5e-3)
+    compare(data, npth.read_state(odata))
+    # the integrator's own storage after the last step: deriv1 = F(y_n),
+    # deriv2 = F(data2) (RK2mid) or F of the ETD1 state (RK2trap), data2 the
+    # midpoint (time_step.py:286, 297, 370, 381)
+    compare(ti.deriv1, npth.read_state(oti.deriv1))
+    compare(ti.deriv2, npth.read_state(oti.deriv2))
+    if integ == 'RK2mid':
+        compare(ti.data2, npth.read_state(oti.data2))
     IF = ti.deriv1['u'][0].integrating_factor
     assert IF is not None and np.allclose(np.asarray(IF), p['nu'] * data['u'][0].k2() ** p.get('viscosity_order', 1))
 

@@ -134,3 +134,68 @@ def test_power_spectrum_bins():
     expect[int(1 / width)] = np.mean([two_pi * 0.09, two_pi * 0.16])   # two samples at |k| = 1
     expect[int(5 / width)] = two_pi * 5 * 0.01
     assert np.allclose(spec, expect, rtol=1e-14, atol=0)
+
+
+def steady_forcing(data, i):
+    """A self-contained forcing function: its source is all a restart has."""
+    import numpy as np
+    f = np.zeros(data['u'][i]['kspace'].shape, dtype=complex)
+    f[1, 2] = 0.05j if i == 0 else -0.02
+    f[2, 1] = 0.03 if i == 0 else 0.04j
+    return f
+
+
+@pytest.mark.parametrize("forced", [False, True], ids=["fused", "forced"])
+def test_restart_continues_bit_exact(forced):
+    """restart(snap_dir) rebuilds (RHS, data, ti) from a snapshot (restart.py:33-69):
+    class, parameters, forcing function rebound by name, fields, clock and
+    stopping controls; running it to the stop iteration equals the
+    uninterrupted run bit for bit."""
+    m = mods()
+    from dedalus.utils.restart import restart
+    case("restart", "forced" if forced else "fused")
+    RHS, data = hydro(m, nu=1.5e-3)
+    RHS.parameters['viscosity_order'] = 2
+    if forced:
+        RHS.set_velocity_forcing(steady_forcing)
+    ti = m.RK2mid(RHS)
+    ti.save_cadence = 3
+    ti.stop_iteration = 9
+    dt = 1e-2
+    while ti.ok:
+        ti.advance(data, dt)
+    final = [np.array(data['u'][i]['kspace']) for i in range(2)]
+    assert sorted(glob.glob('snap_*')) == ['snap_00000', 'snap_00001', 'snap_00002']
+
+    RHS2, data2, ti2 = restart('snap_00002')
+    assert type(RHS2) is m.IncompressibleHydro and type(ti2) is m.RK2mid
+    assert RHS2.parameters['nu'] == 1.5e-3 and RHS2.parameters['viscosity_order'] == 2
+    assert ti2.iteration == 6 and ti2.stop_iteration == 9 and ti2.save_cadence == 3
+    assert data2.time == pytest.approx(6 * dt, abs=1e-15)
+    if forced:
+        assert RHS2.forcing_functions['VelocityForcing'].__name__ == 'steady_forcing'
+    else:
+        assert not RHS2.forcing_functions
+    while ti2.ok:
+        ti2.advance(data2, dt)
+    assert ti2.iteration == 9
+    for i in range(2):
+        assert np.array_equal(np.asarray(data2['u'][i]['kspace']), final[i])
+    # the restarted run continues the snapshot numbering
+    assert os.path.isdir('snap_00003')
+
+
+def test_restart_missing_forcing_raises(tmp_path):
+    """A snapshot naming a forcing function its source does not define
+    cannot be restarted (restart.py:56-59)."""
+    m = mods()
+    from dedalus.utils.restart import restart
+    case("restart, missing forcing")
+    RHS, data = hydro(m)
+    RHS.set_velocity_forcing(steady_forcing)
+    ti = m.RK2mid(RHS)
+    ti.advance(data, 1e-2)
+    with open('snap_00000/forcing_functions.py', 'w') as f:
+        f.write("def other(data, i):\n    return 0\n")
+    with pytest.raises(ValueError):
+        restart('snap_00000')

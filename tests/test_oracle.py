@@ -415,3 +415,220 @@ def test_ensemble_member_initial_condition():
         x = c['xspace'].copy()
         assert abs(np.mean(x ** 2) - mean_square([u[0]], shape)) < 1e-12
         assert not np.allclose(vorticity_member(shape, L, 1001)[0], u[0])
+
+
+# --- a10/a14: cross products and MHD / Boussinesq invariants ------------------------
+
+def _full_spectrum(a, shape):
+    """Full coefficient array indexed by frequency index (y, x) in 2D or
+    (y, z, x) in 3D (each mod n) from a reference-layout half spectrum
+    ((kx, ky) in 2D, (ky, kz, kx) in 3D) of a real field."""
+    if len(shape) == 2:
+        ny, nx = shape
+        f = np.zeros((ny, nx), complex)
+        for ix in range(nx // 2 + 1):
+            for iy in range(ny):
+                f[iy, ix] = a[ix, iy]
+                f[(-iy) % ny, (-ix) % nx] = np.conj(a[ix, iy])
+        return f
+    nz, ny, nx = shape
+    f = np.zeros((ny, nz, nx), complex)
+    for iy in range(ny):
+        for iz in range(nz):
+            for ix in range(nx // 2 + 1):
+                f[iy, iz, ix] = a[iy, iz, ix]
+                f[(-iy) % ny, (-iz) % nz, (-ix) % nx] = np.conj(a[iy, iz, ix])
+    return f
+
+
+def _direct_products(shape, X, Y, combine):
+    """Truncated convolution of band-limited fields: out_k = sum_{p+q=k}
+    combine(X_p, Y_q) over kept p, q, k (2/3 rule), as reference-layout half
+    spectra.  X, Y: lists of component arrays (reference layout)."""
+    nd = len(shape)
+    n = shape[-1]
+    freq = (np.fft.fftfreq(n) * n).astype(int)
+    cut = int(np.ceil(n / 3.0)) - 1        # |f| < n/3 kept
+    fx = [_full_spectrum(a, shape) for a in X]
+    fy = [_full_spectrum(a, shape) for a in Y]
+    kept = [f for f in freq if abs(f) <= cut]
+    import itertools
+    modes = list(itertools.product(kept, repeat=nd))
+    ncomp = len(combine([0.0] * len(X), [0.0] * len(Y)))
+    c = npth.Component(shape, (TWO_PI,) * nd)
+    out = [np.zeros(c.kshape, complex) for _ in range(ncomp)]
+    for p in modes:
+        xp = [f[tuple(v % n for v in p)] for f in fx]
+        for q in modes:
+            k = [a + b for a, b in zip(p, q)]
+            if any(abs(v) > cut for v in k):
+                continue
+            if k[-1] < 0:                   # kx >= 0 half spectrum only
+                continue
+            yq = [f[tuple(v % n for v in q)] for f in fy]
+            val = combine(xp, yq)
+            if nd == 2:
+                idx = (k[1], k[0] % n)       # (kx, ky)
+            else:
+                idx = (k[0] % n, k[1] % n, k[2])   # (ky, kz, kx)
+            for i in range(ncomp):
+                out[i][idx] += val[i]
+    return out
+
+
+def _mhd_physics(shape):
+    ph = npth.IncompressibleMHD(shape)
+    ph.aux = npth.State(shape, ph.length, ph.aux_list)
+    return ph
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8)], ids=str)
+def test_cross_product_3d_equals_truncated_convolution(shape):
+    """XcrossY (physics.py:309-354) of band-limited 3D vectors: every
+    component against the direct sum of u_p x B_q.  Frequency triples are
+    (y, z, x) in the reference's 3D k order."""
+    ph = _mhd_physics(shape)
+    u = [random_k(shape, 70 + i) for i in range(3)]
+    B = [random_k(shape, 80 + i) for i in range(3)]
+    data = npth.load_state(ph, {'u': u, 'B': B})
+    out = ph.create_fields()
+    ph.XcrossY(data['u'], data['B'], out['u'])
+    got = [out['u'][i]['kspace'].copy() for i in range(3)]
+
+    # components in (y, z, x) frequency order map to vector components
+    # x = index 2, y = index 0, z = index 1 of the triple; the vectors here
+    # are (x, y, z) component lists
+    def cross(a, b):
+        return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    want = _direct_products(shape, u, B, cross)
+    for i in range(3):
+        assert rel_l2(got[i], want[i]) < 1e-13, i
+
+
+def test_cross_products_2d_equal_truncated_convolution():
+    """The 2D forms of XcrossY: scalar J x vector B -> (-J B_y, J B_x), and
+    vector u x vector B -> u_x B_y - u_y B_x (physics.py:330-345)."""
+    shape = (8, 8)
+    ph = _mhd_physics(shape)
+    J = random_k(shape, 90)
+    u = [random_k(shape, 91 + i) for i in range(2)]
+    B = [random_k(shape, 93 + i) for i in range(2)]
+    data = npth.load_state(ph, {'u': u, 'B': B})
+    js = ph.aux['mathscalar']
+    js['kspace'] = J
+    out = ph.create_fields()
+    ph.XcrossY(js, data['B'], out['u'])
+    got = [out['u'][i]['kspace'].copy() for i in range(2)]
+    want = _direct_products(shape, [J], B, lambda a, b: [-a[0] * b[1], a[0] * b[0]])
+    for i in range(2):
+        assert rel_l2(got[i], want[i]) < 1e-13, i
+    data = npth.load_state(ph, {'u': u, 'B': B})
+    e = ph.aux['mathscalar']
+    ph.XcrossY(data['u'], data['B'], e)
+    want = _direct_products(shape, u, B, lambda a, b: [a[0] * b[1] - a[1] * b[0]])
+    assert rel_l2(e['kspace'], want[0]) < 1e-13
+
+
+def _energy_drift(kind, shape, kf, weights, params, T=0.2, dt0=1e-2):
+    """|E(T) - E(0)| for dt0 and dt0/2, E = sum_f weight_f <|f|^2> / 2."""
+    def E(fields):
+        return sum(w * energy(fields[name], shape) for name, w in weights.items())
+    e0 = E(kf)
+    errs = []
+    for dt in (dt0, dt0 / 2):
+        out = npth.run(kind, shape, kf, dt, int(round(T / dt)), **dict(params))
+        errs.append(abs(E(out) - e0))
+    return e0, errs
+
+
+def _solenoidal(shape, seed, rms=1.0):
+    ks = project(shape, [random_k(shape, seed + i, slope=1.0) for i in range(len(shape))])[0]
+    s = rms * np.sqrt(0.5 / energy(ks, shape))
+    return [a * s for a in ks]
+
+
+@pytest.mark.parametrize("shape", [(32, 32), (12, 12, 12)], ids=str)
+def test_mhd_inviscid_energy_conservation(shape):
+    """nu = eta = 0: the truncated MHD system conserves |u|^2/2 + |B|^2/(8 pi rho0)
+    (the Lorentz work u.(J x B) cancels the induction term's J.(u x B));
+    RK2's error is O(dt^3) per step.  A wrong sign or factor of the Lorentz
+    force (physics.py:794-802) or of the induction curl (:808-815) breaks it.
+    The drift over the fixed time T falls 4x per halving of dt (3.5 allows
+    for the next order)."""
+    rho0 = 1.7
+    kf = {'u': _solenoidal(shape, 100), 'B': _solenoidal(shape, 110, rms=3.0)}
+    fb = 1.0 / (4 * np.pi * rho0)
+    dt0 = 1e-2 if len(shape) == 2 else 2.5e-3
+    e0, errs = _energy_drift('mhd', shape, kf, {'u': 1.0, 'B': fb}, {'rho0': rho0}, 0.1, dt0)
+    # the kinetic-magnetic exchange over T is a sizeable part of e0
+    out = npth.run('mhd', shape, kf, 1e-2, 20, rho0=rho0)
+    assert abs(energy(out['u'], shape) - energy(kf['u'], shape)) > 1e-3 * e0
+    assert errs[0] < 1e-4 * e0
+    assert errs[1] < errs[0] / 3.5       # O(dt^3) per step: O(dt^2) over T
+
+
+def test_mhd_energy_conservation_fails_for_wrong_lorentz_sign():
+    """The invariant above detects a flipped Lorentz force: with the force
+    reversed the total drifts at first order."""
+    shape = (32, 32)
+    kf = {'u': _solenoidal(shape, 100), 'B': _solenoidal(shape, 110, rms=3.0)}
+    fb = 1.0 / (4 * np.pi)
+    e0, errs = _energy_drift('mhd', shape, kf, {'u': 1.0, 'B': fb}, {'rho0': 1.0})
+    bad = _energy_drift('mhd', shape, kf, {'u': 1.0, 'B': fb}, {'rho0': -1.0})[1]
+    assert bad[1] > 100 * errs[1]
+
+
+@pytest.mark.parametrize("shape,direction", [((32, 32), 'y'), ((12, 12, 12), 'z')], ids=str)
+def test_boussinesq_inviscid_energy_conservation(shape, direction):
+    """nu = kappa = 0: |u|^2/2 + (g alpha_t / beta) T^2 / 2 is conserved (the
+    buoyancy work g alpha_t <u_dir T> cancels the stratification term
+    -beta <T u_dir> of physics.py:705-708); weights chosen unequal so a
+    dropped factor fails."""
+    g, alpha, beta = 2.0, 1.0, 0.5
+    nd = len(shape)
+    kf = {'u': _solenoidal(shape, 120), 'T': [random_k(shape, 130, slope=1.0)]}
+    s = np.sqrt(0.5 / energy(kf['T'], shape))
+    kf['T'] = [kf['T'][0] * s]
+    p = dict(g=g, alpha_t=alpha, beta=beta, boussinesq_direction=direction)
+    e0, errs = _energy_drift('boussinesq', shape, kf, {'u': 1.0, 'T': g * alpha / beta}, p)
+    assert errs[0] < 1e-4 * e0
+    assert errs[1] < errs[0] / 3.5       # O(dt^3) per step: O(dt^2) over T
+    # the weight is not 1: with it the total drifts
+    e1, wrong = _energy_drift('boussinesq', shape, kf, {'u': 1.0, 'T': 1.0}, p)
+    assert wrong[1] > 100 * errs[1]
+
+
+# --- physics.py:575, 697, 805: the projection follows the exact class -----------------
+
+@pytest.mark.parametrize("kind,shape", [('hydro', (16, 16)), ('boussinesq', (16, 16)),
+                                        ('mhd', (8, 8, 8))], ids=str)
+def test_subclass_rhs_is_unprojected(kind, shape):
+    """A subclass that inherits RHS gets the tendency before the pressure
+    projection (the reference tests self.__class__ == <class>): projecting
+    its velocity tendency gives the base class's, and the other fields'
+    tendencies are the base class's."""
+    base = {'hydro': npth.IncompressibleHydro, 'boussinesq': npth.BoussinesqHydro,
+            'mhd': npth.IncompressibleMHD}[kind]
+    sub = type('Sub', (base,), {})
+    nd = len(shape)
+    kf = {'u': [random_k(shape, 140 + i, slope=1.0) for i in range(nd)]}
+    if kind == 'boussinesq':
+        kf['T'] = [random_k(shape, 150, slope=1.0)]
+    if kind == 'mhd':
+        kf['B'] = [random_k(shape, 160 + i, slope=1.0) for i in range(nd)]
+    outs = []
+    for cls in (base, sub):
+        ph = cls(shape) if kind != 'boussinesq' else cls(shape, None, 'y')
+        data = npth.load_state(ph, kf)
+        d = ph.create_fields()
+        ph.RHS(data, d)
+        outs.append(npth.read_state(d))
+    b, s = outs
+    div = project(shape, s['u'])[0]
+    assert max(rel_l2(a, c) for a, c in zip(s['u'], b['u'])) > 1e-3    # not projected
+    for a, c in zip(div, b['u']):
+        assert rel_l2(a, c) < 1e-13
+    for name in b:
+        if name != 'u':
+            for a, c in zip(s[name], b[name]):
+                assert rel_l2(a, c) < 1e-14
