@@ -245,3 +245,42 @@ def test_volume_averages():
     d = 1j * oc.k['x'] * kf['u'][0] + 1j * oc.k['y'] * kf['u'][1]
     d_ref = 2 * d[1:, :].sum() + d[0, :].sum()
     assert abs(vs.known_analysis['divergence'](data, vs.scratch) - d_ref) < 1e-12 * abs(d_ref)
+
+
+@pytest.mark.parametrize("kind,shape,p", [
+    ('hydro', (32, 32), {'nu': 2e-3}),
+    ('boussinesq', (32, 32), {'nu': 1e-3, 'kappa': 1e-3, 'boussinesq_direction': 'y'}),
+    ('mhd', (16, 16, 16), {'nu': 1e-3, 'eta': 1e-3}),
+], ids=str)
+def test_subclass_runs_reference_operator_sequence(kind, shape, p):
+    """A user subclass that inherits RHS gets the reference's class-dependent
+    behaviour (physics.py:575, 697, 805: no pressure projection): RHS and
+    three RK2mid steps against the oracle's subclass of the same class."""
+    m = mods()
+    case("subclass", kind, shape)
+    base = {'hydro': m.IncompressibleHydro, 'boussinesq': m.BoussinesqHydro,
+            'mhd': m.IncompressibleMHD}[kind]
+    obase = {'hydro': npth.IncompressibleHydro, 'boussinesq': npth.BoussinesqHydro,
+             'mhd': npth.IncompressibleMHD}[kind]
+    Sub = type('Sub', (base,), {})
+    OSub = type('OSub', (obase,), {})
+    RHS = Sub(shape, m.FourierRepresentation)
+    RHS.parameters.update(p)
+    kf = kfields(kind, shape, 70)
+    data = RHS.create_fields(0.)
+    set_fields(data, kf)
+    deriv = RHS.create_fields(0.)
+    RHS.RHS(data, deriv)
+    q = dict(p)
+    oph = OSub(shape) if kind != 'boussinesq' else OSub(shape, None, q.pop('boussinesq_direction'))
+    oph.parameters.update(q)
+    odata = npth.load_state(oph, kf)
+    oderiv = oph.create_fields()
+    oph.RHS(odata, oderiv)
+    compare(deriv, npth.read_state(oderiv))
+    ti = m.RK2mid(RHS)
+    oti = npth.RK2mid(oph)
+    for _ in range(3):
+        ti.advance(data, 5e-3)
+        oti.do_advance(odata, 5e-3)
+    compare(data, npth.read_state(odata))
