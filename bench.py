@@ -8,6 +8,10 @@ E(k) ~ k^4 exp(-2 (k/4)^2), rms 1 each, nu = eta = 1e-3, dt = 2.5e-4.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
+Workloads: mhd3d_256 (default), hydro3d_128, boussinesq2d_512,
+taylor_green_64, ensemble_768 (768 FP32 2D members in one graph per step),
+closure_conv (the closure network's convolutions alone, K passes).
+
 Fields of the JSON line besides the standard ones:
   value         whole-job throughput of the timed region (K steps, state
                 resident in HBM, one HIP graph per step replayed K times)
@@ -25,13 +29,19 @@ Fields of the JSON line besides the standard ones:
                 profiles/ for this workload and precision) x K / wall of
                 the timed region, against 8 TB/s; null without a summary
   cpu_baseline  the numpy oracle (a restatement of the reference's numpy
-                path) timed on this host for one step of the same workload
+                path) timed on this host: one warm-up step and the median
+                of three timed steps of the same workload, one thread
   closure       (default workload) the metric's second part: the learned
                 sub-grid closure on 512^2 -> 64^2 forced 2D turbulence --
-                spectrum error of coarse runs with and without it against
-                the coarse-grained fine run, throughput of coarse closure
-                runs (1 and batched members) against the plain 512^2 run,
-                and the MFMA rate of its convolutions
+                spectrum and enstrophy errors of coarse runs with and
+                without it against the coarse-grained fine run (trained on
+                the configuration's 2000 pairs, and on 4000 beside it),
+                the speedup over the plain 512^2 FP64 run for single runs
+                and for both batched at 64 members, and the MFMA rate of
+                its convolutions
+  source_hash   SHA-256 prefix of the kernel sources; rocprof summaries in
+                profiles/ carry the hash of the sources they measured, and
+                their figures are used only when it matches
 """
 
 import argparse
@@ -86,7 +96,9 @@ def gpu_run(w, steps, warmup, precision="f64"):
             state.append(b)
     stage = [plan.alloc_k() for _ in state]
     d1 = [plan.alloc_k() for _ in state]
-    d2 = [plan.alloc_k() for _ in state]
+    # ddx_step does not store the second right-hand side without deriv2
+    # buffers (the reference's deriv2 is scratch to the step itself)
+    d2 = None
     p = w["params"]
     phys = {"hydro": _hip.HYDRO, "boussinesq": _hip.BOUSSINESQ, "mhd": _hip.MHD}[w["kind"]]
     bdir = {"x": 0, "y": 1, "z": 2}[p.get("boussinesq_direction", "z" if nd == 3 else "y")]
@@ -174,29 +186,33 @@ def ensemble_cpu_baseline():
         ph = npth.make_physics("hydro", shape, L, nu=ENSEMBLE_768["nu"])
         data = npth.load_state(ph, {"u": u0})
         ti = npth.RK2mid(ph)
-        ti.do_advance(data, ENSEMBLE_768["dt"])      # first call sets up transforms
-        t0 = time.perf_counter()
-        reps = 5
-        for _ in range(reps):
+        ti.do_advance(data, ENSEMBLE_768["dt"])      # warm-up step
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
             ti.do_advance(data, ENSEMBLE_768["dt"])
-        tot_t += (time.perf_counter() - t0) / reps * n
+            ts.append(time.perf_counter() - t0)
+        tot_t += float(np.median(ts)) * n
         tot_pts += shape[0] * shape[1] * n
     return {
         "value": tot_pts / tot_t / 1e6,
         "unit": "Mpt-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": "5 RK2mid steps of one member per grid size (64^2, 128^2, 256^2) on the "
-                  "numpy oracle (FP64, single thread), weighted by 256 members each; host %s, "
-                  "%d logical CPUs, numpy %s" % (cpu_model(), os.cpu_count() or 0, np.__version__),
+        "sample": "one member per grid size (64^2, 128^2, 256^2) on the numpy oracle (FP64, one "
+                  "thread): 1 warm-up + 5 timed RK2mid steps, median, weighted by 256 members "
+                  "each; %s" % host_description(),
     }
 
 
-def closure_summary(batches=(1, 64, 1024), iters=None, t_eval=10.0):
+def closure_summary(batches=(1, 64), t_eval=20.0):
     """The learned closure on 512^2 -> 64^2 forced 2D turbulence: pairs from
-    the fine solver, training, spectrum error of coarse runs against the
-    coarse-grained fine run, throughput against the plain 512^2 run, and the
-    MFMA rate of the network's forward convolutions."""
+    the fine solver, training on the configuration's 2000 pairs (and, beside
+    it, on all 4000 generated), spectrum and enstrophy errors of coarse runs
+    against the coarse-grained fine run over t in [5, 10] and [10, 20],
+    throughput against the plain 512^2 run (single runs, and both batched at
+    the same member count), and the MFMA rate of the network's forward
+    convolutions."""
     import numpy as np
     import torch
     from dedalus.closure import ClosureNet, pipeline as P
@@ -206,25 +222,42 @@ def closure_summary(batches=(1, 64, 1024), iters=None, t_eval=10.0):
     X, Y, info = pipe.generate_pairs()
     t_pairs = time.perf_counter() - t0
     log("closure: %d pairs in %.1f s" % (X.shape[0], t_pairs))
-    net = ClosureNet(cin=3, cout=2, width=cfg["width"], depth=cfg["depth"], seed=0).cuda()
-    iters = iters or cfg["train_iters"]
-    t1 = time.perf_counter()
-    scales, hist = P.train(net, X, Y, iters=iters)
-    t_train = time.perf_counter() - t1
-    log("closure: trained %d iterations in %.1f s" % (iters, t_train))
-    res = P.evaluate(pipe, net, scales, T=t_eval, seeds=cfg["eval_seeds"])
-    log("closure: spectrum error %s" % res["err"])
-    t_fine = P.time_fine_steps(pipe)
+    nseeds = len(cfg["train_seeds"])
+    sets = {"closure_%d" % cfg["train_pairs"]: P.train_subset(X, Y, nseeds, cfg["train_pairs"]),
+            "closure_%d" % X.shape[0]: (X, Y)}
+    main = "closure_%d" % cfg["train_pairs"]
+    iters = cfg["train_iters"]
+    nets, train = {}, {}
+    for label, (XX, YY) in sets.items():
+        net = ClosureNet(cin=3, cout=2, width=cfg["width"], depth=cfg["depth"], seed=0).cuda()
+        t1 = time.perf_counter()
+        scales, hist = P.train(net, XX, YY, iters=iters)
+        train[label] = {"pairs": int(XX.shape[0]), "train_s": round(time.perf_counter() - t1, 1),
+                        "final_loss": round(hist[-1][1], 4), "val_loss": round(hist[-1][2], 4)}
+        log("closure: %s trained %d iterations in %.1f s" % (label, iters, train[label]["train_s"]))
+        nets[label] = (net, scales)
+    windows = [(t_eval / 4, t_eval / 2), (t_eval / 2, t_eval)]
+    res = P.evaluate(pipe, nets=nets, T=t_eval, seeds=cfg["eval_seeds"], windows=windows)
+    log("closure: spectrum error %s, enstrophy error %s" % (res["errs"], res["enstrophy_errs"]))
     ratio = cfg["dt_ratio"]
     thr = {}
     for b in batches:
-        r = pipe.coarse_run(net, scales, batch=None if b == 1 else b)
+        t_fine = P.time_fine_batched(pipe, b, 200 if b == 1 else 50)
+        r = pipe.coarse_run(*nets[main], batch=None if b == 1 else b)
         r.capture()
-        t = P.time_steps(pipe, r, 50 if b > 64 else 200)
-        thr[str(b)] = {"ms_per_step": round(t, 4),
-                       "speedup_vs_plain_512": round(ratio * t_fine * b / t, 2)}
+        t = P.time_steps(pipe, r, 200 if b == 1 else 50)
+        plain = pipe.coarse_run(None, batch=None if b == 1 else b)
+        plain.capture()
+        t_plain = P.time_steps(pipe, plain, 200 if b == 1 else 50)
+        thr[str(b)] = {"coarse_closure_ms_per_step": round(t, 4),
+                       "coarse_plain_ms_per_step": round(t_plain, 4),
+                       "fine_ms_per_step": round(t_fine, 4),
+                       "speedup": round(ratio * t_fine / t, 2)}
+    r = lambda d: {k: round(v, 4) for k, v in d.items()}
+    sp1, sp64 = thr["1"]["speedup"], thr[str(batches[-1])]["speedup"]
     # MFMA rate of the forward convolutions on 256 coarse images
     from dedalus.closure.net import Inference
+    net = nets[main][0]
     B, (H, W) = 256, pipe.coarse_shape
     with torch.cuda.stream(pipe.stream):
         inf = Inference(net, B, H, W)
@@ -242,16 +275,27 @@ def closure_summary(batches=(1, 64, 1024), iters=None, t_eval=10.0):
     alg = sum(2 * 9 * ch[l] * ch[l + 1] for l in range(len(ch) - 1)) * B * H * W
     padded = net.flops_per_pixel() * B * H * W
     return {
-        "config": "512^2 -> 64^2 (8x8 box), nu 1e-4, dt 2.5e-4 / 2e-3, forcing |k| = 4 rms 0.1",
-        "pairs": int(X.shape[0]), "pairs_s": round(t_pairs, 1),
-        "train_iters": iters, "train_s": round(t_train, 1),
-        "final_loss": round(hist[-1][1], 4), "val_loss": round(hist[-1][2], 4),
-        "spectrum_err": {k: round(v, 4) for k, v in res["err"].items()},
-        "spectrum_err_def": "mean over 1 <= k <= %d of |E_model/E_truth - 1|, shell sums "
-                            "averaged over %d held-out seeds and t in [%g, %g]"
-                            % (res["kc"], len(res["seeds"]), t_eval / 2, t_eval),
-        "fine_ms_per_step": round(t_fine, 4),
+        "config": "512^2 -> 64^2 (8x8 box), nu 1e-4, dt 2.5e-4 / 2e-3, forcing |k| = 4 rms 0.1; "
+                  "network 3-32-32-32-32-2, %d parameters" % sum(p.numel() for p in net.parameters()),
+        "pairs_generated": int(X.shape[0]), "pairs_s": round(t_pairs, 1),
+        "train_pairs": cfg["train_pairs"], "train_iters": iters, "train": train,
+        "spectrum_err": r(res["errs"][0]),
+        "enstrophy_err": r(res["enstrophy_errs"][0]),
+        "spectrum_err_t10_20": r(res["errs"][1]),
+        "enstrophy_err_t10_20": r(res["enstrophy_errs"][1]),
+        "spectrum_err_per_seed": {k: [round(x, 4) for x in v] for k, v in res["member_errs"][0].items()},
+        "spectrum_err_def": "mean over 1 <= k <= %d of |E_model/E_truth - 1|, Hermitian shell sums "
+                            "averaged over %d held-out seeds and t in [%g, %g] (also [%g, %g]); "
+                            "enstrophy <w^2>/2 (volume_average.py:191-200), |Z/Z_truth - 1| of the "
+                            "same averages; training set for the headline: the configuration's "
+                            "%d pairs" % (res["kc"], len(res["seeds"]), windows[0][0], windows[0][1],
+                                         windows[1][0], windows[1][1], cfg["train_pairs"]),
         "throughput": thr,
+        "speedup_def": "simulated time per wall second of the coarse closure run over the plain "
+                       "512^2 FP64 run, dt ratio %d x t_fine / t_coarse; '1': one run each (the "
+                       "metric's comparison), '%d': %d members of each batched on one plan "
+                       "(like for like)" % (ratio, batches[-1], batches[-1]),
+        "meets_20x": {"single_run": bool(sp1 >= 20.0), "batched_like_for_like": bool(sp64 >= 20.0)},
         "conv_roofline": {"bound": "mfma", "kernel": "conv3x3 forward (5 layers, 256 images 64^2)",
                           "achieved": round(alg / (ms * 1e-3) / 1e12, 2),
                           "peak": 2500.0, "unit": "TFLOP/s",
@@ -261,10 +305,58 @@ def closure_summary(batches=(1, 64, 1024), iters=None, t_eval=10.0):
     }
 
 
+def closure_conv_run(batch=256, size=64, reps=20):
+    """The closure network's convolutions alone (kernel profiles): the
+    preallocated inference pass and one training forward + backward of the
+    5-layer width-32 network on `batch` images of size^2."""
+    import torch
+    from dedalus import _hip
+    from dedalus.closure.net import ClosureNet, Inference
+    _hip.init()
+    B, H = batch, size
+    net = ClosureNet(cin=3, cout=2, width=32, depth=5, seed=0).cuda()
+    inf = Inference(net, B, H, H)
+    inf.x.normal_()
+    x = torch.randn(B, H, H, 3, device="cuda")
+    inf.run()
+    net(x).square().mean().backward()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    e[0].record()
+    for _ in range(reps):
+        inf.run()
+    e[1].record()
+    for _ in range(reps):
+        net.zero_grad(set_to_none=True)
+        net(x).square().mean().backward()
+    e[2].record()
+    e[2].synchronize()
+    fwd_ms = e[0].elapsed_time(e[1]) / reps
+    train_ms = e[1].elapsed_time(e[2]) / reps
+    ch = net.chans
+    alg = sum(2 * 9 * ch[l] * ch[l + 1] for l in range(len(ch) - 1)) * B * H * H
+    return {"metric": "closure convolution TFLOP/s (bf16 MFMA, fp32 accumulate)",
+            "value": round(alg / fwd_ms / 1e9, 2), "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": reps, "higher_is_better": True, "dtype": "bf16",
+            "data": "synthetic: random images and seeded weights",
+            "config": {"workload": "closure_conv", "batch": B, "size": H,
+                       "channels": list(ch)},
+            "forward_ms": round(fwd_ms, 4), "train_step_ms": round(train_ms, 4),
+            "train_TFLOPs": round(3 * alg / train_ms / 1e9, 2),
+            "roofline": {"bound": "mfma", "achieved": round(alg / fwd_ms / 1e9, 2), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(alg / fwd_ms / 1e9 / 2500.0, 5),
+                         "traffic": None}}
+
+
 def step_traffic_from_profiles(workload, precision):
-    """HBM bytes per step from the committed rocprofv3 step summary, if any."""
+    """HBM bytes per step from the committed rocprofv3 step summary, if any,
+    and only if it was measured on the kernels built from the current
+    sources (its source_hash line)."""
     path = os.path.join(ROOT, "profiles", "step_%s_hbm_traffic.txt" % workload)
     if not os.path.exists(path):
+        return None
+    if profile_hash(path) != source_hash():
+        log("warning: %s was measured on other kernel sources; hbm_measured is null" % path)
         return None
     key = "hbm_bytes_per_step_%s" % precision
     with open(path) as f:
@@ -279,6 +371,9 @@ def traffic_from_profiles(kernel, workload):
     path = os.path.join(ROOT, "profiles", "%s_%s_pmc.txt" % (kernel, workload))
     if not os.path.exists(path):
         return None
+    if profile_hash(path) != source_hash():
+        log("warning: %s was measured on other kernel sources; traffic is null" % path)
+        return None
     with open(path) as f:
         for line in f:
             if line.startswith("hbm_bytes_per_launch"):
@@ -286,31 +381,74 @@ def traffic_from_profiles(kernel, workload):
     return None
 
 
-def cpu_baseline(w):
-    """One RK2mid step of the same workload on the numpy oracle, one process."""
+def host_description():
+    import numpy as np
+    try:
+        import scipy
+        sv = scipy.__version__
+    except ImportError:
+        sv = "absent"
+    return "host %s, %d logical CPUs, numpy %s, scipy %s" % (cpu_model(), os.cpu_count() or 0,
+                                                             np.__version__, sv)
+
+
+def cpu_baseline(w, timed=3):
+    """The numpy oracle (the reference's numpy path restated) on this host,
+    one process: one warm-up RK2mid step of the same workload, then `timed`
+    steps; the median step gives Mpt-steps/s.  numpy.fft (pocketfft) and the
+    ufuncs run on one thread."""
     sys.path.insert(0, ROOT)
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
     import numpy as np
     from oracle import numpy_path as npth
 
     ph = npth.make_physics(w["kind"], w["shape"], w["length"], **dict(w["params"]))
     data = npth.load_state(ph, w["fields"])
     ti = (npth.RK2mid if w["integrator"] == "RK2mid" else npth.RK2trap)(ph)
-    log("cpu baseline: one oracle step")
-    t0 = time.perf_counter()
+    log("cpu baseline: 1 warm-up + %d timed oracle steps" % timed)
     ti.do_advance(data, w["dt"])
-    t1 = time.perf_counter()
+    ts = []
+    for i in range(timed):
+        t0 = time.perf_counter()
+        ti.do_advance(data, w["dt"])
+        ts.append(time.perf_counter() - t0)
+        log("cpu baseline: step %d %.2f s" % (i + 1, ts[-1]))
+    med = float(np.median(ts))
     npts = float(np.prod(w["shape"]))
     return {
-        "value": npts / (t1 - t0) / 1e6,
+        "value": npts / med / 1e6,
         "unit": "Mpt-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": "one RK2mid step of the same %s workload (%s) on the numpy oracle, "
-                  "single thread (pocketfft); %.1f s; host %s, %d logical CPUs, numpy %s"
-                  % (w["name"], "x".join(map(str, w["shape"])), t1 - t0, cpu_model(),
-                     os.cpu_count() or 0, np.__version__),
+        "sample": "RK2mid steps of the same %s workload (%s) on the numpy oracle, one process, "
+                  "one thread (pocketfft): 1 warm-up step, then %d timed steps %s s, median "
+                  "%.2f s/step; %s"
+                  % (w["name"], "x".join(map(str, w["shape"])), timed,
+                     "/".join("%.2f" % t for t in ts), med, host_description()),
     }
+
+
+def source_hash():
+    """First 16 hex digits of SHA-256 over the kernel sources and headers
+    (dedalus-1.0_amd/csrc/*, include/*.h), in name order."""
+    import hashlib
+    h = hashlib.sha256()
+    for d, pat in (("dedalus-1.0_amd/csrc", None), ("include", ".h")):
+        base = os.path.join(ROOT, d)
+        for name in sorted(os.listdir(base)):
+            if pat and not name.endswith(pat):
+                continue
+            with open(os.path.join(base, name), "rb") as f:
+                h.update(name.encode())
+                h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_hash(path):
+    with open(path) as f:
+        for line in f:
+            if line.startswith("source_hash"):
+                return line.split("=", 1)[1].strip()
+    return None
 
 
 def main():
@@ -334,6 +472,9 @@ def main():
               "got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "closure_conv":
+        print(json.dumps(closure_conv_run(reps=args.steps)), flush=True)
+        return
     if args.workload == "ensemble_768":
         from dedalus.ensemble import ENSEMBLE_768
         wall, prof, ens = ensemble_run(args.steps, args.warmup)
@@ -417,6 +558,7 @@ def main():
         "step_roofline": step_roof,
         "hbm_measured": hbm_meas,
         "kernels": kernels,
+        "source_hash": source_hash(),
     }
     if not args.no_closure and w["name"] == "mhd3d_256":
         try:
