@@ -40,10 +40,23 @@
 
 using namespace ddx;
 
+// Device scratch of ddx_step_noalloc: one copy per GPU, resident from the
+// library's first use on that GPU until the process ends.  Host code never
+// passes `scratch` itself to a kernel (on the host it names a stand-in, not
+// a device address); it uses the address hipGetSymbolAddress returns for
+// each device (scratch_base below).
+#define DDX_SCRATCH_BYTES (64u << 20)
+static __device__ __align__(256) unsigned char scratch[DDX_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
 int g_device = -1;
+
+// per-device address of `scratch`, resolved by ddx_init with the device current
+constexpr int kMaxDevices = 64;
+constexpr size_t kScratchBytes = DDX_SCRATCH_BYTES;
+void* g_scratch[kMaxDevices] = {};
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -285,6 +298,16 @@ int ensure_f2(ddx_plan* p, int ns) {
   p->f2_bytes = need;
   return 0;
 }
+
+// Device address of the static scratch on `device` (set up by ddx_init).
+int scratch_base(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices || !g_scratch[device])
+    return fail(DDX_ERR_NODEV, "ddx_init has not been called for this plan's device");
+  *out = g_scratch[device];
+  return 0;
+}
+
+size_t scratch_round(size_t n) { return (n + 255) / 256 * 256; }
 
 // Device scratch of one right-hand side (two buffers of the mixed layouts)
 // and, for ddx_step without deriv2 buffers and unfused sweeps, the second
@@ -1160,6 +1183,13 @@ int ddx_init(int device) {
     return fail(DDX_ERR_NODEV, "no HIP device available");
   if (device < 0 || device >= n) return fail(DDX_ERR_ARG, "device index out of range");
   HIPCHK(hipSetDevice(device));
+  if (device < kMaxDevices && !g_scratch[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(DDX_ERR_HIP, std::string("scratch symbol address: ") + hipGetErrorString(e));
+    g_scratch[device] = ptr;
+  }
   g_device = device;
   return 0;
 }
@@ -1601,6 +1631,36 @@ int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* st
              void* const* stage, void* const* deriv1, void* const* deriv2,
              const void* const* forcing, int nsteps, double dt) {
   return step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, nsteps, dt, nullptr);
+}
+
+int ddx_step_noalloc(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
+                     void* const* stage, void* const* deriv1, void* const* deriv2,
+                     const void* const* forcing, int nsteps, double dt) {
+  if (check_plan(p) || !prm) return fail(DDX_ERR_ARG, "null argument");
+  const int nf = rhs_ws_fields(prm->physics, p->ndim);
+  const int ns = nstate_of(prm, p->ndim);
+  if (nf < 0 || ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
+  void* base = nullptr;
+  int rc = scratch_base(p->device, &base);
+  if (rc) return rc;
+  // A | B | F2 (F2 only when the sweeps run unfused and no deriv2 is given)
+  const size_t half = scratch_round((size_t)nf * ws_field_stride(p) * cpx_bytes(p));
+  const size_t f2 = (!deriv2 && !p->fuse_update) ? scratch_round((size_t)ns * p->nk * cpx_bytes(p)) : 0;
+  const size_t need = 2 * half + f2;
+  if (need > kScratchBytes)
+    return fail(DDX_ERR_ARG, "ddx_step_noalloc: this grid and physics need " + std::to_string(need) +
+                                 " bytes of scratch; the static scratch holds " +
+                                 std::to_string(kScratchBytes) + " (use ddx_step)");
+  Ws ws;
+  ws.A = base;
+  ws.B = (unsigned char*)base + half;
+  ws.F2 = f2 ? (unsigned char*)base + 2 * half : nullptr;
+  // per-launch timing would create HIP events: off for this call
+  const bool prof = p->prof;
+  p->prof = false;
+  rc = step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, nsteps, dt, &ws);
+  p->prof = prof;
+  return rc;
 }
 
 extern "C++" {

@@ -91,6 +91,7 @@ SIGNATURES = {
     "ddx_update": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _D, _I, _D]),
     "ddx_rhs": (_I, [_VP, ctypes.POINTER(Params), _PP, _PP, _PP]),
     "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
+    "ddx_step_noalloc": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
     "ddx_kcombine": (_I, [_VP, _VP, _I, _PP, _DP, ctypes.POINTER(_I), _I]),
     "ddx_xcombine": (_I, [_VP, _VP, _I, _PP, _PP, _DP, _I]),
     "ddx_xreduce": (_I, [_VP, _VP, _I, _DP]),
@@ -442,6 +443,20 @@ class Plan:
                              ptr_array([int(s) for s in state]), st,
                              ptr_array([int(d) for d in deriv1]), d2,
                              fz, int(nsteps), float(dt)))
+
+
+    def step_noalloc(self, integrator, params, state, stage, deriv1, deriv2, nsteps, dt,
+                     forcing=None):
+        """ddx_step_noalloc: ddx_step on the library's static device scratch."""
+        st = ptr_array([int(s) for s in stage]) if stage is not None else None
+        d2 = ptr_array([int(d) for d in deriv2]) if deriv2 is not None else None
+        fz = None
+        if forcing is not None:
+            fz = ptr_array([int(f) if f is not None else None for f in forcing])
+        _check(_lib.ddx_step_noalloc(self.handle, integrator, ctypes.byref(params),
+                                     ptr_array([int(s) for s in state]), st,
+                                     ptr_array([int(d) for d in deriv1]), d2,
+                                     fz, int(nsteps), float(dt)))
 
 
 def make_params(physics, nu=0., viscosity_order=1, kappa=0., eta=0., g=1., alpha_t=1.,

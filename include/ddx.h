@@ -243,6 +243,23 @@ int ddx_step(ddx_plan* plan, int integrator, const ddx_params* params, void* con
              void* const* stage, void* const* deriv1, void* const* deriv2,
              const void* const* forcing, int nsteps, double dt);
 
+/* ddx_step that allocates nothing: no hipMalloc, hipFree or other allocating
+ * call, and no internal graph capture (the kernels are launched directly on
+ * the plan's stream, so a caller may capture the call in a graph of its own).
+ * No counterpart in the reference; it is for callers that cannot allocate:
+ * code under HIP graph capture and tight serving loops.  The right-hand
+ * side's workspace (and, when the ETD sweeps run unfused and deriv2 is
+ * NULL, the second derivative) lives in a static device array of 64 MiB per
+ * GPU, held from ddx_init until the process ends.  A grid and physics that
+ * need more (e.g. 2D up to 512^2, 3D hydro up to 96^3 in FP64 fit) fail with
+ * DDX_ERR_ARG and a message naming the size; use ddx_step for them.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams, two threads) race.  Arguments and results are those of ddx_step;
+ * per-launch profiling is not recorded during the call. */
+int ddx_step_noalloc(ddx_plan* plan, int integrator, const ddx_params* params, void* const* state,
+                     void* const* stage, void* const* deriv1, void* const* deriv2,
+                     const void* const* forcing, int nsteps, double dt);
+
 /* ---- instrumentation (no reference counterpart; replaces the wall-clock
  *      Timer of dedalus/utils/timer.py:29-65 at kernel granularity) ------- */
 
