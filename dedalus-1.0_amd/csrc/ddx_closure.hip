@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
+#include <type_traits>
 #include <string>
 
 #include "../../include/ddx.h"
@@ -797,6 +799,265 @@ int check_conv(int B, int H, int W, int Ci, int Co) {
 
 inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ---------------------------------------------------------------------------
+// The whole network in one launch (inference)
+// ---------------------------------------------------------------------------
+//
+// NL layers of 3x3 periodic convolution: 16 (padded) input channels, HID
+// hidden channels with GELU, 16 (padded) linear output channels.  A
+// workgroup computes one TX x TY output tile of one image: it stages the
+// tile's input with a halo of NL pixels in LDS, then each layer reads its
+// input region from one LDS buffer and writes its output region, NL - 1 - l
+// pixels of halo narrower, to the other.  Only the network's input and
+// output touch HBM; the halo is recomputed by neighbouring tiles.
+//
+// Every layer is the same arithmetic as conv3x3_tile_kernel's (tap loop
+// t = 0..8, K in the same chunks, bias then GELU, bf16 activations), with
+// the MFMA operands swapped: A = weights (16 output channels x K), B = 16
+// pixels' input channels, so a lane's D values are four consecutive output
+// channels of one pixel (one 8-byte LDS write).
+constexpr int NET_MAXL = 6;
+
+struct NetArgs {
+  const uint16_t* w[NET_MAXL];    // layer l: [Cop][9][Cip] bf16
+  const float* bias[NET_MAXL];    // layer l: [Cop] fp32
+  const uint16_t* x;              // [npix][16] bf16, or null: the fields below
+  const double* src[DDX_CONV_MAXCH];
+  float in_scale[DDX_CONV_MAXCH];
+  int cin;
+  float* out;                     // [npix][16] fp32, or null: the fields below
+  double* dst[DDX_CONV_MAXCH];
+  const double* add[DDX_CONV_MAXCH];  // all null, or one per output channel
+  float out_scale[DDX_CONV_MAXCH];
+  int cout;
+  int B, H, W;
+};
+
+// LDS plan of one tile: buffer 0 holds the input and the outputs of layers
+// 1, 3, ...; buffer 1 the outputs of layers 0, 2, ...  Pixel strides of 16 + 4
+// and HID + 4 bf16 (8-byte aligned rows, consecutive pixels on distinct banks).
+template <int NL, int HID, int TX, int TY>
+struct NetPlan {
+  static constexpr int PSI = 16 + 4, PSH = HID + 4;
+  static constexpr int in_bytes() { return (TX + 2 * NL) * (TY + 2 * NL) * PSI * 2; }
+  static constexpr int out_bytes(int l) {
+    return (TX + 2 * (NL - 1 - l)) * (TY + 2 * (NL - 1 - l)) * PSH * 2;
+  }
+  static constexpr int buf_bytes(int b) {
+    int m = b == 0 ? in_bytes() : 0;
+    for (int l = 0; l < NL - 1; ++l)
+      if ((l + 1) % 2 == b && out_bytes(l) > m) m = out_bytes(l);
+    return (m + 15) / 16 * 16;
+  }
+  static constexpr int lds_bytes() { return buf_bytes(0) + buf_bytes(1); }
+};
+
+__device__ inline bf16x8 gld8(const uint16_t* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const u32x4 u = {v.x, v.y, v.z, v.w};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ inline bf16x4 gld4(const uint16_t* p) { return load4(p); }
+
+__device__ inline int wrapm(int i, int n) {
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// One layer over an output region of w x h pixels (row-major, flattened into
+// 16-pixel fragments; the last fragment's spare lanes repeat the last pixel
+// and store nothing).  in: (w + 2) x (h + 2) pixels with stride psi;
+// LAST layers write the tile's pixels to global memory.
+template <int CI, int CO, bool LAST, int NT>
+__device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* w_l, const float* b_l,
+                                          const uint16_t* in, int psi, uint16_t* outb, int pso,
+                                          int w, int h, int b, int y0, int x0) {
+  constexpr int NCO = CO / 16;
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  // weights as A operands, all nine taps, straight from global memory (L2)
+  typedef typename std::conditional<CI == 32, bf16x8, bf16x4>::type wvec;
+  wvec wv[9][NCO];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int j = 0; j < NCO; ++j) {
+      const uint16_t* q = w_l + ((j * 16 + m) * 9 + t) * CI;
+      if constexpr (CI == 32) wv[t][j] = gld8(q + 8 * kq);
+      else wv[t][j] = gld4(q + 4 * kq);
+    }
+  float bv[NCO][4];
+#pragma unroll
+  for (int j = 0; j < NCO; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bv[j][i] = b_l[j * 16 + 4 * kq + i];
+  const int npx = w * h, nfrag = (npx + 15) / 16;
+  const int wi = w + 2;
+  for (int f0 = 2 * wave; f0 < nfrag; f0 += 2 * NW) {
+    // two fragments per pass: four independent accumulation chains per wave
+    int pp[2], base[2];
+    bool valid[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      int p = (f0 + u) * 16 + m;
+      valid[u] = p < npx;
+      if (!valid[u]) p = npx - 1;
+      const int r = p / w, c = p - r * w;
+      pp[u] = p;
+      base[u] = r * wi + c;
+    }
+    f32x4 acc[2][NCO];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int j = 0; j < NCO; ++j) acc[u][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int off = (t / 3) * wi + (t % 3);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const uint16_t* q = in + (base[u] + off) * psi;
+        if constexpr (CI == 32) {
+          const bf16x8 xv = lds8(q + 8 * kq);
+#pragma unroll
+          for (int j = 0; j < NCO; ++j)
+            acc[u][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[t][j], xv, acc[u][j], 0, 0, 0);
+        } else {
+          const bf16x4 xv = lds4(q + 4 * kq);
+#pragma unroll
+          for (int j = 0; j < NCO; ++j)
+            acc[u][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv[t][j], xv, acc[u][j], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (!LAST) {
+        if (!valid[u]) continue;
+#pragma unroll
+        for (int j = 0; j < NCO; ++j) {
+          uint32_t h2[2];
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            const float v0 = gelu(acc[u][j][2 * e] + bv[j][2 * e]);
+            const float v1 = gelu(acc[u][j][2 * e + 1] + bv[j][2 * e + 1]);
+            h2[e] = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+          }
+          *reinterpret_cast<uint2*>(&outb[pp[u] * pso + j * 16 + 4 * kq]) = make_uint2(h2[0], h2[1]);
+        }
+      } else {
+        if (!valid[u]) continue;
+        const int r = pp[u] / w, c = pp[u] - r * w;
+        const long gp = ((long)b * a.H + y0 + r) * a.W + x0 + c;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = acc[u][0][i] + bv[0][i];
+        if (a.out) {
+          *reinterpret_cast<float4*>(&a.out[gp * 16 + 4 * kq]) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int co = 4 * kq + i;
+            if (co < a.cout) {
+              const float x = v[i] * a.out_scale[co];
+              a.dst[co][gp] = (double)x + (a.add[0] ? a.add[co][gp] : 0.0);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int L, int NL, int HID, int TX, int TY, int NT>
+__device__ __forceinline__ void net_layers(const NetArgs& a, uint16_t* buf0, uint16_t* buf1, int b,
+                                           int y0, int x0) {
+  if constexpr (L < NL) {
+    using P = NetPlan<NL, HID, TX, TY>;
+    constexpr int CI = L == 0 ? 16 : HID;
+    constexpr int CO = L == NL - 1 ? 16 : HID;
+    constexpr int g = NL - 1 - L;  // output halo
+    const uint16_t* in = (L % 2 == 0) ? buf0 : buf1;
+    uint16_t* outb = (L % 2 == 0) ? buf1 : buf0;
+    net_layer<CI, CO, L == NL - 1, NT>(a, a.w[L], a.bias[L], in, L == 0 ? P::PSI : P::PSH, outb,
+                                       P::PSH, TX + 2 * g, TY + 2 * g, b, y0, x0);
+    if constexpr (L < NL - 1) {
+      lds_barrier();
+      net_layers<L + 1, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0);
+    }
+  }
+}
+
+template <int NL, int HID, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) conv_net_kernel(NetArgs a) {
+  using P = NetPlan<NL, HID, TX, TY>;
+  extern __shared__ __align__(16) uint16_t netlds[];
+  uint16_t* buf0 = netlds;
+  uint16_t* buf1 = netlds + P::buf_bytes(0) / 2;
+  const int tiles_x = a.W / TX, tiles_y = a.H / TY;
+  int rest = blockIdx.x;
+  const int tx = rest % tiles_x;
+  rest /= tiles_x;
+  const int ty = rest % tiles_y;
+  const int b = rest / tiles_y;
+  const int x0 = tx * TX, y0 = ty * TY;
+  // stage the input region with a halo of NL pixels (periodic wrap)
+  constexpr int RW = TX + 2 * NL, RH = TY + 2 * NL;
+  for (int i = threadIdx.x; i < RW * RH; i += NT) {
+    const int r = i / RW, c = i - r * RW;
+    const int yy = wrapm(y0 - NL + r, a.H), xx = wrapm(x0 - NL + c, a.W);
+    const long gp = ((long)b * a.H + yy) * a.W + xx;
+    uint4 q0, q1;
+    if (a.x) {
+      q0 = *reinterpret_cast<const uint4*>(&a.x[gp * 16]);
+      q1 = *reinterpret_cast<const uint4*>(&a.x[gp * 16 + 8]);
+    } else {
+      float v[16];
+#pragma unroll
+      for (int ch = 0; ch < 16; ++ch) {
+        const int cc = ch < a.cin ? ch : 0;
+        v[ch] = ch < a.cin ? (float)a.src[cc][gp] * a.in_scale[cc] : 0.f;
+      }
+      q0 = make_uint4((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                      (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16),
+                      (uint32_t)f2bf(v[4]) | ((uint32_t)f2bf(v[5]) << 16),
+                      (uint32_t)f2bf(v[6]) | ((uint32_t)f2bf(v[7]) << 16));
+      q1 = make_uint4((uint32_t)f2bf(v[8]) | ((uint32_t)f2bf(v[9]) << 16),
+                      (uint32_t)f2bf(v[10]) | ((uint32_t)f2bf(v[11]) << 16),
+                      (uint32_t)f2bf(v[12]) | ((uint32_t)f2bf(v[13]) << 16),
+                      (uint32_t)f2bf(v[14]) | ((uint32_t)f2bf(v[15]) << 16));
+    }
+    uint2* d = reinterpret_cast<uint2*>(&buf0[i * P::PSI]);
+    d[0] = make_uint2(q0.x, q0.y);
+    d[1] = make_uint2(q0.z, q0.w);
+    d[2] = make_uint2(q1.x, q1.y);
+    d[3] = make_uint2(q1.z, q1.w);
+  }
+  lds_barrier();
+  net_layers<0, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0);
+}
+
+template <int NL, int HID, int TX, int TY, int NT>
+int launch_net(const NetArgs& a, hipStream_t s) {
+  using P = NetPlan<NL, HID, TX, TY>;
+  constexpr int lds = P::lds_bytes();
+  static_assert(lds <= 160 * 1024, "tile does not fit the LDS");
+  static bool attr = false;
+  if (!attr) {
+    const hipError_t e = hipFuncSetAttribute((const void*)conv_net_kernel<NL, HID, TX, TY, NT>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess)
+      return ddx::set_error(DDX_ERR_HIP, std::string("conv_net LDS attribute: ") + hipGetErrorString(e));
+    attr = true;
+  }
+  const long ntiles = (long)a.B * (a.H / TY) * (a.W / TX);
+  hipLaunchKernelGGL((conv_net_kernel<NL, HID, TX, TY, NT>), dim3((unsigned)ntiles), dim3(NT), lds, s, a);
+  CL_LAUNCHCHK("conv_net");
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -990,6 +1251,68 @@ int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void
                        (const float*)src, (float*)dst, ny, nx, factor);
   CL_LAUNCHCHK("box_filter");
   return 0;
+}
+
+int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
+                 const void* const* w, const float* const* bias, const void* x, float* out,
+                 const void* const* src, const float* in_scale, void* const* dst,
+                 const void* const* add, const float* out_scale) {
+  if (B < 1 || H < 1 || W < 1) return ddx::set_error(DDX_ERR_ARG, "bad image shape");
+  if (!chans || !w || !bias) return ddx::set_error(DDX_ERR_ARG, "null argument");
+  if ((!x && !src) || (!out && !dst)) return ddx::set_error(DDX_ERR_ARG, "no input or no output");
+  if (nl < 2 || nl > NET_MAXL) return ddx::set_error(DDX_ERR_UNSUPPORTED, "network depth");
+  const int cin = chans[0], cout = chans[nl], hid = chans[1];
+  if (cin < 1 || cin > 16 || cout < 1 || cout > 16)
+    return ddx::set_error(DDX_ERR_UNSUPPORTED, "network input and output need <= 16 channels");
+  for (int l = 1; l < nl; ++l)
+    if (chans[l] != hid) return ddx::set_error(DDX_ERR_UNSUPPORTED, "hidden layers of one width");
+  NetArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int l = 0; l < nl; ++l) {
+    if (!w[l] || !bias[l]) return ddx::set_error(DDX_ERR_ARG, "null weights or bias");
+    if ((uintptr_t)w[l] % 16) return ddx::set_error(DDX_ERR_ARG, "weights must be 16-byte aligned");
+    a.w[l] = (const uint16_t*)w[l];
+    a.bias[l] = bias[l];
+  }
+  if (x && (uintptr_t)x % 16) return ddx::set_error(DDX_ERR_ARG, "x must be 16-byte aligned");
+  if (out && (uintptr_t)out % 16) return ddx::set_error(DDX_ERR_ARG, "out must be 16-byte aligned");
+  a.x = (const uint16_t*)x;
+  a.out = out;
+  a.cin = cin;
+  a.cout = cout;
+  if (!x) {
+    for (int c = 0; c < cin; ++c) {
+      if (!src[c]) return ddx::set_error(DDX_ERR_ARG, "null input field");
+      a.src[c] = (const double*)src[c];
+      a.in_scale[c] = in_scale ? in_scale[c] : 1.f;
+    }
+  }
+  if (!out) {
+    for (int c = 0; c < cout; ++c) {
+      if (!dst[c] || (add && !add[c])) return ddx::set_error(DDX_ERR_ARG, "null output field");
+      a.dst[c] = (double*)dst[c];
+      a.add[c] = add ? (const double*)add[c] : nullptr;
+      a.out_scale[c] = out_scale ? out_scale[c] : 1.f;
+    }
+  }
+  a.B = B;
+  a.H = H;
+  a.W = W;
+  hipStream_t s = (hipStream_t)stream;
+  // large tiles (less recomputed halo) when they alone fill the chip, small
+  // tiles (more workgroups) for a few images
+  const long big = (H % 16 == 0 && W % 32 == 0) ? (long)B * (H / 16) * (W / 32) : 0;
+  const bool small_ok = H % 8 == 0 && W % 16 == 0;
+  if (nl == 5 && hid == 32) {
+    if (big >= 256 || (big > 0 && !small_ok)) return launch_net<5, 32, 32, 16, 512>(a, s);
+    if (small_ok) return launch_net<5, 32, 16, 8, 256>(a, s);
+  } else if (nl == 3 && hid == 16) {
+    if (big >= 256 || (big > 0 && !small_ok)) return launch_net<3, 16, 32, 16, 256>(a, s);
+    if (small_ok) return launch_net<3, 16, 16, 8, 256>(a, s);
+  }
+  return ddx::set_error(DDX_ERR_UNSUPPORTED,
+                        "fused network: depth 5 width 32 or depth 3 width 16 on images of "
+                        "multiples of 8 x 16 pixels");
 }
 
 int ddx_mfma_probe(void* stream, int k, const void* A, const void* Bm, float* D) {

@@ -108,6 +108,8 @@ SIGNATURES = {
     "ddx_conv_pack": (_I, [_VP, ctypes.c_int64, _I, _I, _PP, ctypes.c_int64, _I, _VP, _VP, _VP]),
     "ddx_conv_unpack": (_I, [_VP, ctypes.c_int64, _I, _I, _VP, _PP, ctypes.c_int64, _I, _VP, _PP]),
     "ddx_box_filter": (_I, [_VP, _I, _I, _I, _I, _VP, _VP]),
+    "ddx_conv_net": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(_I), _PP, _PP, _VP, _VP, _PP, _VP,
+                          _PP, _PP, _VP]),
     "ddx_mfma_probe": (_I, [_VP, _I, _VP, _VP, _VP]),
 }
 

@@ -265,9 +265,14 @@ class Inference:
     """Forward pass of a ClosureNet on fixed-size batches with preallocated
     buffers: ``run()`` launches only the convolutions (no allocation, no host
     synchronisation), so it can be captured in a graph.  Call ``refresh()``
-    after the weights change."""
+    after the weights change.
 
-    def __init__(self, net, B, H, W):
+    fused: the whole network as one launch (``ddx_conv_net``: activations
+    stay on chip) where the kernel supports the architecture and image size
+    (None: whenever it does), else one ``ddx_conv3x3`` launch per layer; both
+    give the same values."""
+
+    def __init__(self, net, B, H, W, fused=None):
         self.net = net
         self.shape = (int(B), int(H), int(W))
         self.npix = B * H * W
@@ -276,14 +281,30 @@ class Inference:
         dev = net.weights[0].device
         ch = net.chans
         self.x = torch.zeros((self.npix, pad16(ch[0])), dtype=torch.bfloat16, device=dev)
-        self.acts = [torch.empty((self.npix, pad16(c)), dtype=torch.bfloat16, device=dev)
-                     for c in ch[1:-1]]
         self.out = torch.empty((self.npix, pad16(ch[-1])), dtype=torch.float32, device=dev)
         self.wf = [torch.empty((pad16(ch[l + 1]), 9, pad16(ch[l])), dtype=torch.bfloat16, device=dev)
                    for l in range(len(ch) - 1)]
         self.bp = [torch.zeros(pad16(ch[l + 1]), dtype=torch.float32, device=dev)
                    for l in range(len(ch) - 1)]
+        nl = len(ch) - 1
+        self._chans = (ctypes.c_int * (nl + 1))(*ch)
+        self._wptr = _hip.ptr_array([t.data_ptr() for t in self.wf])
+        self._bptr = _hip.ptr_array([t.data_ptr() for t in self.bp])
+        self.fused = fused
+        if self.fused is None:
+            self.fused = self._fused_supported()
+        self.acts = None if self.fused else [
+            torch.empty((self.npix, pad16(c)), dtype=torch.bfloat16, device=dev) for c in ch[1:-1]]
         self.refresh()
+
+    def _fused_supported(self):
+        """The fused kernel's shape rule (include/ddx_closure.h, ddx_conv_net)."""
+        ch = self.net.chans
+        nl = len(ch) - 1
+        B, H, W = self.shape
+        hid = set(ch[1:-1])
+        return ((nl, hid) in ((5, {32}), (3, {16})) and ch[0] <= 16 and ch[-1] <= 16
+                and H % 8 == 0 and W % 16 == 0)
 
     def refresh(self):
         lib = _lib()
@@ -298,6 +319,10 @@ class Inference:
     def run(self):
         B, H, W = self.shape
         nl = len(self.net.chans) - 1
+        if self.fused:
+            _check(_lib().ddx_conv_net(_stream(), B, H, W, nl, self._chans, self._wptr, self._bptr,
+                                       _ptr(self.x), _ptr(self.out), None, None, None, None, None))
+            return self.out
         a = self.x
         for l in range(nl):
             if l < nl - 1:
@@ -308,3 +333,19 @@ class Inference:
                 conv3x3(a, self.wf[l], B, H, W, bias=self.bp[l], mode=_hip.CONV_LINEAR,
                         out_f32=self.out, out_bf16=False)
         return self.out
+
+    def run_fields(self, src, in_scale, dst, out_scale, add=None):
+        """The network from solver fields to solver fields in one launch:
+        input channel c = bf16(in_scale[c] * src[c]) (float64 x-space arrays,
+        as ``pack`` rounds them), output dst[c] = out_scale[c] * y_c (+ add[c])
+        (as ``unpack``).  Needs the fused kernel."""
+        if not self.fused:
+            raise _hip.DDXError(-3, "run_fields needs the fused network kernel")
+        B, H, W = self.shape
+        nl = len(self.net.chans) - 1
+        isc = (ctypes.c_float * len(in_scale))(*[float(v) for v in in_scale])
+        osc = (ctypes.c_float * len(out_scale))(*[float(v) for v in out_scale])
+        ad = _hip.ptr_array([int(d) for d in add]) if add is not None else None
+        _check(_lib().ddx_conv_net(_stream(), B, H, W, nl, self._chans, self._wptr, self._bptr,
+                                   None, None, _hip.ptr_array([int(d) for d in src]), isc,
+                                   _hip.ptr_array([int(d) for d in dst]), ad, osc))

@@ -223,8 +223,8 @@ class Pipeline:
         return X, Y, info
 
     # ---- coarse runs -----------------------------------------------------
-    def coarse_run(self, net=None, scales=None, batch=None):
-        return CoarseRun(self, net, scales, batch)
+    def coarse_run(self, net=None, scales=None, batch=None, fused=None):
+        return CoarseRun(self, net, scales, batch, fused)
 
 
 def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=None):
@@ -270,7 +270,7 @@ class CoarseRun:
     plan (one set of launches for all members, the network on B images).
     ``capture()`` records one step as a graph on the pipeline's stream."""
 
-    def __init__(self, pipe, net=None, scales=None, batch=None):
+    def __init__(self, pipe, net=None, scales=None, batch=None, fused=None):
         self.pipe = pipe
         self.batch = int(batch) if batch else None
         if self.batch:
@@ -300,7 +300,7 @@ class CoarseRun:
             self.in_scale = [1.0 / float(v) for v in xs]
             self.out_scale = [float(v) for v in ys]
             with torch.cuda.stream(pipe.stream):
-                self.inf = Inference(net, nb, m[0], m[1])
+                self.inf = Inference(net, nb, m[0], m[1], fused=fused)
             self.xb = [C.alloc_x() for _ in range(3)]
             self.fx = [C.alloc_x() for _ in range(2)]
             self.fk = [C.alloc_k() for _ in range(2)]
@@ -327,11 +327,15 @@ class CoarseRun:
         forcing = self.forcing
         if self.net is not None:
             features_x(C, self.state, self.xb, self.wk)
-            pack([(int(b), True) for b in self.xb], self.npix, self.inf.x.shape[1],
-                 scale=self.in_scale, bf16=self.inf.x, f32=False)
-            out = self.inf.run()
-            unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale,
-                   add=[int(b) for b in self.fcx])
+            if self.inf.fused:
+                # features -> network -> tendency + coarse forcing, one launch
+                self.inf.run_fields(self.xb, self.in_scale, self.fx, self.out_scale, add=self.fcx)
+            else:
+                pack([(int(b), True) for b in self.xb], self.npix, self.inf.x.shape[1],
+                     scale=self.in_scale, bf16=self.inf.x, f32=False)
+                out = self.inf.run()
+                unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale,
+                       add=[int(b) for b in self.fcx])
             C.fwd_many(self.fx, self.fk)
             forcing = self.fk
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,

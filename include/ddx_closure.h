@@ -93,6 +93,25 @@ int ddx_conv_unpack(void* stream, int64_t npix, int C, int Cp, const float* src,
  * (factor*Y, factor*X); src is ny x nx (float64 if f64, else float32). */
 int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void* src, void* dst);
 
+/* The whole network's forward pass in one launch (inference): nl 3x3
+ * periodic convolution layers with chans[0..nl] channels (padded to 16 in
+ * the weights and activations), GELU after every layer but the last, bf16
+ * activations and fp32 accumulation -- the arithmetic of nl ddx_conv3x3
+ * calls (GELU mode, then LINEAR), without the intermediate activations
+ * leaving the chip.  w[l] = layer l's [Cop][9][Cip] bf16 weights (16-byte
+ * aligned), bias[l] its [Cop] fp32 bias.  Input: x [npix][16] bf16, or (x
+ * NULL) chans[0] float64 fields src[c][p] scaled by in_scale[c] and rounded
+ * to bf16 as ddx_conv_pack does.  Output: out [npix][16] fp32, or (out NULL)
+ * chans[nl] float64 fields dst[c][p] = out_scale[c] * y_c[p] (+ add[c][p])
+ * as ddx_conv_unpack does.  Supported: depth 5 with 32 hidden channels and
+ * depth 3 with 16, inputs and outputs of at most 16 channels, H a multiple
+ * of 8 and W of 16; otherwise DDX_ERR_UNSUPPORTED and the layer-by-layer
+ * calls apply.  in_scale, out_scale, add may be NULL (1, 1, nothing). */
+int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
+                 const void* const* w, const float* const* bias, const void* x, float* out,
+                 const void* const* src, const float* in_scale, void* const* dst,
+                 const void* const* add, const float* out_scale);
+
 /* Layout check: D = A B for one 16 x 16 x k bf16 tile (k = 16 or 32) on one
  * wavefront, with the lane placement the convolution kernels use (A 16 x k,
  * B k x 16 row-major bf16, D 16 x 16 row-major fp32). */

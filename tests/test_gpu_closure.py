@@ -314,3 +314,64 @@ def test_inference_matches_network_forward():
     out = inf.run()
     torch.cuda.synchronize()
     assert torch.equal(out[:, :2].reshape(B, H, W, 2), ref)
+
+
+@pytest.mark.parametrize("B,H,W,width,depth", [(1, 64, 64, 32, 5), (3, 64, 64, 32, 5),
+                                                (256, 64, 64, 32, 5), (2, 32, 32, 16, 3),
+                                                (1, 16, 48, 32, 5)], ids=str)
+def test_fused_network_matches_layers(B, H, W, width, depth):
+    """ddx_conv_net (the whole network in one launch, activations in LDS,
+    large or small tiles with recomputed halos) equals the layer-by-layer
+    ddx_conv3x3 calls bit for bit."""
+    d = dev()
+    from dedalus.closure import ClosureNet
+    from dedalus.closure.net import Inference, pack
+    case("fused network", B, H, W, width, depth)
+    net = ClosureNet(cin=3, cout=2, width=width, depth=depth, seed=7).to(d)
+    g = torch.Generator().manual_seed(17)
+    with torch.no_grad():
+        for b in net.biases:
+            b.copy_((torch.randn(b.shape, generator=g) * 0.1).to(d))
+    x = torch.randn(B, H, W, 3, generator=g).to(d)
+    outs = []
+    for fused in (True, False):
+        inf = Inference(net, B, H, W, fused=fused)
+        assert inf.fused == fused
+        pack([(x.data_ptr() + 4 * c, False) for c in range(3)], B * H * W, inf.x.shape[1],
+             pstride=3, bf16=inf.x, f32=False)
+        outs.append(inf.run().clone())
+    torch.cuda.synchronize()
+    assert torch.isfinite(outs[0]).all()
+    assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("B", [1, 4])
+def test_fused_network_from_fields(B):
+    """run_fields (solver fields in, scaled tendency plus forcing out, one
+    launch) equals pack -> layer-by-layer network -> unpack bit for bit."""
+    d = dev()
+    from dedalus.closure import ClosureNet
+    from dedalus.closure.net import Inference, pack, unpack
+    case("fused network from fields", B)
+    H = W = 64
+    net = ClosureNet(cin=3, cout=2, width=32, depth=5, seed=8).to(d)
+    g = torch.Generator().manual_seed(19)
+    with torch.no_grad():
+        for b in net.biases:
+            b.copy_((torch.randn(b.shape, generator=g) * 0.1).to(d))
+    src = [torch.randn(B, H, W, generator=g, dtype=torch.float64).to(d) for _ in range(3)]
+    add = [torch.randn(B, H, W, generator=g, dtype=torch.float64).to(d) for _ in range(2)]
+    isc, osc = [0.7, 1.3, 0.25], [0.05, 0.02]
+    f = Inference(net, B, H, W, fused=True)
+    dst_f = [torch.empty(B, H, W, dtype=torch.float64, device=d) for _ in range(2)]
+    f.run_fields([t.data_ptr() for t in src], isc, [t.data_ptr() for t in dst_f], osc,
+                 add=[t.data_ptr() for t in add])
+    l = Inference(net, B, H, W, fused=False)
+    pack([(t.data_ptr(), True) for t in src], B * H * W, l.x.shape[1], scale=isc, bf16=l.x, f32=False)
+    out = l.run()
+    dst_l = [torch.empty(B, H, W, dtype=torch.float64, device=d) for _ in range(2)]
+    unpack(out, 2, [t.data_ptr() for t in dst_l], True, scale=osc, add=[t.data_ptr() for t in add])
+    torch.cuda.synchronize()
+    for a, b in zip(dst_f, dst_l):
+        assert torch.isfinite(a).all()
+        assert torch.equal(a, b)

@@ -278,6 +278,15 @@ def test_subclass_runs_reference_operator_sequence(kind, shape, p):
     oderiv = oph.create_fields()
     oph.RHS(odata, oderiv)
     compare(deriv, npth.read_state(oderiv))
+    # fresh objects for the run: the first RHS call sets the integrating
+    # factors on the deriv it is given (physics.py:538-540), as in the reference
+    RHS = Sub(shape, m.FourierRepresentation)
+    RHS.parameters.update(p)
+    data = RHS.create_fields(0.)
+    set_fields(data, kf)
+    oph = OSub(shape) if kind != 'boussinesq' else OSub(shape, None, p['boussinesq_direction'])
+    oph.parameters.update(q)
+    odata = npth.load_state(oph, kf)
     ti = m.RK2mid(RHS)
     oti = npth.RK2mid(oph)
     for _ in range(3):
