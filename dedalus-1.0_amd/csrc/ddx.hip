@@ -780,6 +780,42 @@ struct FusedUpdate {
   bool write_deriv;
 };
 
+// Arguments of the k-space epilogue of one right-hand side (and of the ETD
+// sweep fused into it when fu is given); the products U are the caller's.
+template <typename T>
+EpiArgs<T> epi_args(const ddx_plan* p, const ddx_params* prm, int nstate, const cpx<T>* const* state,
+                    cpx<T>* const* deriv, const cpx<T>* const* forcing, const FusedUpdate<T>* fu) {
+  EpiArgs<T> e;
+  memset(&e, 0, sizeof(e));
+  for (int s = 0; s < nstate; ++s) {
+    e.state[s] = state[s];
+    e.deriv[s] = deriv ? deriv[s] : nullptr;
+    e.forcing[s] = forcing ? forcing[s] : nullptr;
+  }
+  e.g = kgeom(p);
+  e.nk = p->nk;
+  e.physics = prm->physics;
+  e.ndim = p->ndim;
+  e.bdir = prm->bdir;
+  e.npts = (double)p->nper;
+  e.galpha = prm->g * prm->alpha_t;
+  e.beta = prm->beta;
+  e.upd_kind = -1;
+  if (fu) {
+    e.upd_kind = fu->kind;
+    e.upd_order = fu->order;
+    e.upd_dt = fu->dt;
+    e.write_deriv = fu->write_deriv ? 1 : 0;
+    for (int s = 0; s < nstate; ++s) {
+      e.upd_y[s] = fu->y[s];
+      e.upd_f1[s] = fu->f1 ? fu->f1[s] : nullptr;
+      e.upd_out[s] = fu->out[s];
+      e.upd_coeff[s] = fu->coeff[s];
+    }
+  }
+  return e;
+}
+
 // Workspace fields a right-hand side of this physics needs in each buffer
 int rhs_ws_fields(int physics, int nd) {
   PhysLayout L;
@@ -902,36 +938,11 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   }
 
   // k-space epilogue
-  EpiArgs<T> e;
-  memset(&e, 0, sizeof(e));
+  EpiArgs<T> e = epi_args<T>(p, prm, L.nstate, state, deriv, forcing, fu);
   for (int o = 0; o < L.np; ++o) e.U[o] = Uc + o * fs;
-  for (int s = 0; s < L.nstate; ++s) {
-    e.state[s] = state[s];
-    e.deriv[s] = deriv ? deriv[s] : nullptr;
-    e.forcing[s] = forcing ? forcing[s] : nullptr;
-  }
-  e.g = kgeom(p);
-  e.nk = p->nk;
-  e.physics = prm->physics;
-  e.ndim = nd;
-  e.bdir = prm->bdir;
-  e.npts = (double)p->nper;
-  e.galpha = prm->g * prm->alpha_t;
-  e.beta = prm->beta;
-  e.upd_kind = -1;
   e.nt = big_nt && DDX_EPI_NT;
   int upd_arrays = 0;
   if (fu) {
-    e.upd_kind = fu->kind;
-    e.upd_order = fu->order;
-    e.upd_dt = fu->dt;
-    e.write_deriv = fu->write_deriv ? 1 : 0;
-    for (int s = 0; s < L.nstate; ++s) {
-      e.upd_y[s] = fu->y[s];
-      e.upd_f1[s] = fu->f1 ? fu->f1[s] : nullptr;
-      e.upd_out[s] = fu->out[s];
-      e.upd_coeff[s] = fu->coeff[s];
-    }
     // y read, out written, F1 read (ETD2RK*), deriv not written (stage 2)
     upd_arrays = 2 + (fu->kind >= DDX_ETD2RK1 ? 1 : 0) - (fu->write_deriv ? 0 : 1);
   }

@@ -450,11 +450,19 @@ __global__ void __launch_bounds__(64) conv3x3_wgrad_kernel(const uint16_t* __res
 // (pixels) is contiguous: A[m = co][k = p] from dzT[co][p] (8-byte reads),
 // B[k = p][n = ci] from haloT[ci][row][col].  Wavefront w owns the
 // (co tile, ci tile) pairs w, w + 4, ... with all nine tap accumulators.
+//
+// With dzf (the fp32 dz), the block also sums dz over its pixels per output
+// channel in double (the bias gradient), thread t taking channel t mod CO and
+// every (256 / CO)-th pixel, combined in a fixed order into
+// bpartial[block][co]: the column sums ride along with the tiles this
+// kernel reads anyway, instead of a separate pass over dz.
 template <int CI, int CO>
 __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t* __restrict__ x,
                                                                  const uint16_t* __restrict__ dz,
                                                                  float* __restrict__ partial,
-                                                                 int B, int H, int W) {
+                                                                 int B, int H, int W,
+                                                                 const float* __restrict__ dzf = nullptr,
+                                                                 double* __restrict__ bpartial = nullptr) {
   constexpr int DS = 260;            // dzT row stride (bf16): 256 pixels + 4
   constexpr int HRS = 68;            // halo row stride (66 columns + 2)
   constexpr int HS = 6 * HRS + 2;    // halo channel stride, odd in 4-byte words
@@ -471,6 +479,9 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t*
   for (int u = 0; u < NPW; ++u)
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[u][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  constexpr int PG = 256 / CO;              // pixel groups of the bias sums
+  const int bco = tid % CO, bpg = tid / CO;
+  double bsum = 0.0;
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     long rest = tile;
     const int tx = (int)(rest % tiles_x);
@@ -478,6 +489,17 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t*
     const int ty = (int)(rest % tiles_y);
     const int b = (int)(rest / tiles_y);
     const int x0 = tx * 64, y0 = ty * 4;
+    if (dzf) {
+      // loads first, then the double adds in pixel order
+      float bv[256 / PG];
+#pragma unroll
+      for (int k = 0; k < 256 / PG; ++k) {
+        const int pix = bpg + PG * k;
+        bv[k] = dzf[(((long)b * H + y0 + (pix >> 6)) * W + x0 + (pix & 63)) * CO + bco];
+      }
+#pragma unroll
+      for (int k = 0; k < 256 / PG; ++k) bsum += (double)bv[k];
+    }
     lds_barrier();  // the previous tile's reads are done
     // dz: 256 pixels x CO, 4 channels per load, stored transposed
     for (int i = tid; i < 256 * (CO / 4); i += 256) {
@@ -537,6 +559,18 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t*
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         partial[(((long)blockIdx.x * CO + co0 + 4 * kq + i) * 9 + t) * CI + ci0 + m] = acc[u][t][i];
+  }
+  if (dzf) {
+    // the PG partial sums of each channel, combined in group order
+    lds_barrier();
+    double* sh = reinterpret_cast<double*>(haloT);   // 256 doubles fit the halo buffer
+    sh[tid] = bsum;
+    lds_barrier();
+    if (tid < CO) {
+      double r = 0.0;
+      for (int g = 0; g < PG; ++g) r += sh[g * CO + tid];
+      bpartial[(long)blockIdx.x * CO + tid] = r;
+    }
   }
 }
 
@@ -1146,6 +1180,43 @@ int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const v
   const long n = (long)Co * 9 * Ci;
   hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
   CL_LAUNCHCHK("conv3x3_wgrad sum");
+  return 0;
+}
+
+int ddx_conv3x3_grads(void* stream, int B, int H, int W, int Ci, int Co, const void* x,
+                      const void* dz, const float* dzf, float* partial, double* bpartial, int nchunk,
+                      float* dw, float* db) {
+  int rc = check_conv(B, H, W, Ci, Co);
+  if (rc) return rc;
+  if (!x || !dz || !dzf || !partial || !bpartial || !dw || !db || nchunk < 1)
+    return ddx::set_error(DDX_ERR_ARG, "null argument");
+  if (!(W % 64 == 0 && H % 4 == 0 && (Ci == 16 || Ci == 32) && (Co == 16 || Co == 32)))
+    return ddx::set_error(DDX_ERR_UNSUPPORTED,
+                          "fused gradients need W % 64 == 0, H % 4 == 0 and 16 or 32 channels");
+  hipStream_t s = (hipStream_t)stream;
+  const long ntiles = (long)B * (H / 4) * (W / 64);
+  const int nch = (int)(ntiles < nchunk ? ntiles : nchunk);
+  const int key = Ci * 100 + Co;
+  const uint16_t* xx = (const uint16_t*)x;
+  const uint16_t* dd = (const uint16_t*)dz;
+  if (key == 1616)
+    hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<16, 16>), dim3(nch), dim3(256), 0, s, xx, dd, partial,
+                       B, H, W, dzf, bpartial);
+  else if (key == 1632)
+    hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<16, 32>), dim3(nch), dim3(256), 0, s, xx, dd, partial,
+                       B, H, W, dzf, bpartial);
+  else if (key == 3216)
+    hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<32, 16>), dim3(nch), dim3(256), 0, s, xx, dd, partial,
+                       B, H, W, dzf, bpartial);
+  else
+    hipLaunchKernelGGL((conv3x3_wgrad_tile_kernel<32, 32>), dim3(nch), dim3(256), 0, s, xx, dd, partial,
+                       B, H, W, dzf, bpartial);
+  CL_LAUNCHCHK("conv3x3 grads");
+  const long n = (long)Co * 9 * Ci;
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
+  CL_LAUNCHCHK("conv3x3 grads: weight sum");
+  hipLaunchKernelGGL(colsum_final_kernel, dim3(Co), dim3(256), 0, s, bpartial, db, nch, Co);
+  CL_LAUNCHCHK("conv3x3 grads: bias sum");
   return 0;
 }
 

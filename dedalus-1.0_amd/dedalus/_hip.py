@@ -103,6 +103,7 @@ SIGNATURES = {
     "ddx_conv3x3": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
     "ddx_conv3x3_wgrad": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP]),
     "ddx_conv_colsum": (_I, [_VP, _VP, ctypes.c_int64, _I, _VP, _I, _VP]),
+    "ddx_conv3x3_grads": (_I, [_VP, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
     "ddx_conv_wprep": (_I, [_VP, _VP, _I, _I, _I, _I, _VP, _VP]),
     "ddx_conv_wunpack": (_I, [_VP, _VP, _I, _I, _I, _I, _VP]),
     "ddx_conv_pack": (_I, [_VP, ctypes.c_int64, _I, _I, _PP, ctypes.c_int64, _I, _VP, _VP, _VP]),

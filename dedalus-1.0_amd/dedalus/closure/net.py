@@ -87,6 +87,28 @@ def wgrad(x_bf16, dz_bf16, B, H, W, nchunk=None):
     return dw
 
 
+def grads(x_bf16, dz_bf16, dz_f32, B, H, W, nchunk=None):
+    """(padded weight gradient [Co, 9, Ci], bias gradient [Co]) of one layer in
+    one pass over dz (ddx_conv3x3_grads), or None where that kernel does not
+    apply."""
+    Ci = x_bf16.shape[1]
+    Co = dz_bf16.shape[1]
+    if not (W % 64 == 0 and H % 4 == 0 and Ci in (16, 32) and Co in (16, 32)):
+        return None
+    lib = _lib()
+    npix = B * H * W
+    if nchunk is None:
+        nchunk = max(1, min(512, npix // 256))
+    dev = x_bf16.device
+    partial = torch.empty((nchunk, Co, 9, Ci), dtype=torch.float32, device=dev)
+    bpartial = torch.empty((nchunk, Co), dtype=torch.float64, device=dev)
+    dw = torch.empty((Co, 9, Ci), dtype=torch.float32, device=dev)
+    db = torch.empty(Co, dtype=torch.float32, device=dev)
+    _check(lib.ddx_conv3x3_grads(_stream(), B, H, W, Ci, Co, _ptr(x_bf16), _ptr(dz_bf16), _ptr(dz_f32),
+                                 _ptr(partial), _ptr(bpartial), int(nchunk), _ptr(dw), _ptr(db)))
+    return dw, db
+
+
 def colsum(a, nchunk=None):
     """Column sums of a [npix, C] float32 tensor (deterministic)."""
     lib = _lib()
@@ -221,8 +243,13 @@ class _NetFn(torch.autograd.Function):
         gws, gbs = [None] * nl, [None] * nl
         for l in range(nl - 1, -1, -1):
             ci, co = chans[l], chans[l + 1]
-            gws[l] = wunpack(wgrad(acts[l], dzb, B, H, W), co, ci)
-            gbs[l] = colsum(dzf)[:co].contiguous()
+            g2 = grads(acts[l], dzb, dzf, B, H, W)
+            if g2 is None:
+                gws[l] = wunpack(wgrad(acts[l], dzb, B, H, W), co, ci)
+                gbs[l] = colsum(dzf)[:co].contiguous()
+            else:
+                gws[l] = wunpack(g2[0], co, ci)
+                gbs[l] = g2[1][:co].contiguous()
             if l > 0:
                 dzf, dzb = conv3x3(dzb, wts[l], B, H, W, aux=zs[l - 1],
                                    mode=_hip.CONV_GELU_GRAD, out_f32=True, out_bf16=True)

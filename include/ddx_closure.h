@@ -58,6 +58,17 @@ int ddx_conv3x3(void* stream, int B, int H, int W, int Ci, int Co, const void* x
 int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const void* x,
                       const void* dz, float* partial, int nchunk, float* dw);
 
+/* Both gradients of one layer in one pass over dz: ddx_conv3x3_wgrad's
+ * dw [Co][9][Ci] and ddx_conv_colsum's db[co] = sum_p dzf[p][co] (dzf the
+ * fp32 dz, dz its bf16 rounding), the column sums taken per block of the
+ * weight-gradient kernel in double and added in a fixed order.  partial:
+ * nchunk * Co * 9 * Ci floats, bpartial: nchunk * Co doubles of scratch.
+ * Widths W % 64 == 0, heights H % 4 == 0 and 16 or 32 channels only
+ * (DDX_ERR_UNSUPPORTED otherwise: call the two separately). */
+int ddx_conv3x3_grads(void* stream, int B, int H, int W, int Ci, int Co, const void* x,
+                      const void* dz, const float* dzf, float* partial, double* bpartial, int nchunk,
+                      float* dw, float* db);
+
 /* out[c] = sum_p a[p][c] (bias gradient), c < C <= 256.  partial is scratch
  * of nchunk * C doubles; pixels are split into at most nchunk chunks summed
  * in a fixed order (deterministic). */

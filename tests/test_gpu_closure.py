@@ -375,3 +375,29 @@ def test_fused_network_from_fields(B):
     for a, b in zip(dst_f, dst_l):
         assert torch.isfinite(a).all()
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("B,H,W,Ci,Co", [(2, 8, 64, 16, 32), (1, 4, 128, 32, 16), (2, 64, 64, 32, 32),
+                                          (3, 12, 64, 16, 16)], ids=str)
+@pytest.mark.parametrize("nchunk", [1, 7, 64])
+def test_conv_grads_one_pass(B, H, W, Ci, Co, nchunk):
+    """ddx_conv3x3_grads (weight and bias gradients in one pass over dz):
+    the weight gradient equals ddx_conv3x3_wgrad's bit for bit, the bias
+    gradient equals the float64 column sum of the fp32 dz to 1e-12 relative
+    (it is summed in double) and ddx_conv_colsum's to float rounding."""
+    d = dev()
+    net = C()
+    case("conv grads one pass", B, H, W, Ci, Co, nchunk)
+    x, _, _ = random_conv(B, H, W, Ci, Co, 12)
+    g = torch.Generator().manual_seed(13)
+    dzf = torch.randn(B * H * W, Co, generator=g)
+    dzb = dzf.to(torch.bfloat16)
+    xd, dzbd, dzfd = x.reshape(-1, Ci).to(d), dzb.to(d), dzf.to(d)
+    dw, db = net.grads(xd, dzbd, dzfd, B, H, W, nchunk=nchunk)
+    dw_ref = net.wgrad(xd, dzbd, B, H, W, nchunk=nchunk)
+    db_ref = net.colsum(dzfd)
+    torch.cuda.synchronize()
+    assert torch.equal(dw, dw_ref)
+    exact = dzf.double().sum(dim=0)
+    assert rl2(db, exact) < 1e-7
+    assert rl2(db, db_ref) < 1e-7
