@@ -169,6 +169,34 @@ def ensemble_run(steps, warmup):
     if not np.all(np.isfinite(e)):
         raise RuntimeError("non-finite ensemble state after the timed steps")
     prof["_wall_instrumented_s"] = t3 - t2
+    # each group alone (its own graph, the same K steps): the concurrent
+    # ensemble step against the sum and the maximum of its parts
+    alone = {}
+    for g in ens.groups:
+        g.plan.set_graph(True)
+        g.step(ens.params, 2, ens.dt)
+        g.plan.sync()
+        ta = time.perf_counter()
+        g.step(ens.params, steps, ens.dt)
+        g.plan.sync()
+        alone["%dx%d^2" % (g.n, g.shape[0])] = round((time.perf_counter() - ta) / steps * 1e3, 4)
+    # the autotuner's choice per group, then the same timed region again
+    ens.seed_members(ENSEMBLE_768["seed0"])
+    chosen = ens.autotune(steps=4, reps=2)
+    ens.capture()
+    ens.seed_members(ENSEMBLE_768["seed0"])
+    ens.advance(warmup)
+    ens.sync()
+    t4 = time.perf_counter()
+    ens.advance(steps)
+    ens.sync()
+    tuned = (time.perf_counter() - t4) / steps * 1e3
+    prof["_ensemble"] = {"groups_alone_ms_per_step": alone,
+                         "sum_of_groups_ms": round(sum(alone.values()), 4),
+                         "max_group_ms": round(max(alone.values()), 4),
+                         "builtin_ms_per_step": round((t1 - t0) / steps * 1e3, 4),
+                         "tuned_ms_per_step": round(tuned, 4),
+                         "tuned_choice": {"%dx%d" % k: v for k, v in chosen.items()}}
     return t1 - t0, prof, ens
 
 
@@ -492,6 +520,7 @@ def main():
             npts *= n
         wall, prof = gpu_run(w, args.steps, args.warmup, args.precision)
     wall_instr = prof.pop("_wall_instrumented_s")
+    ens_info = prof.pop("_ensemble", None)
     if args.calibrate_copy_mib:
         from dedalus import _hip
         n = args.calibrate_copy_mib << 20
@@ -560,6 +589,8 @@ def main():
         "kernels": kernels,
         "source_hash": source_hash(),
     }
+    if ens_info:
+        out["ensemble"] = ens_info
     if not args.no_closure and w["name"] == "mhd3d_256":
         try:
             out["closure"] = closure_summary()

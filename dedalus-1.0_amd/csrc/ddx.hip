@@ -178,7 +178,7 @@ struct ddx_plan {
   // (DDX_NO_ZSPLIT=1 forms every derived field before the z pass)
   bool zsplit = true;
   // ddx_plan_set_tuning choices, -1: the built-in rule
-  int tune_wide = -1, tune_xcd = -1, tune_yepi = -1;
+  int tune_wide = -1, tune_xcd = -1, tune_yepi = -1, tune_radix = -1;
   int nbatch = 1;
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
@@ -337,14 +337,17 @@ int tile_width(int n, size_t cb) {
 
 enum ColKind { COL_PLAIN, COL_SPECTRAL };
 
-template <typename T, int N, bool WIDE>
+template <typename T, int N, bool WIDE, int V>
 void col_fast_go(ddx_plan* p, const ColArgs<T>& a, dim3 grid, int dir, bool spectral) {
-  const dim3 block(ColPlan<N>::LANES * col_width<N, WIDE>());
+  const dim3 block(ColPlanV<N, V>::LANES * col_width<N, WIDE, V>());
   // spectral prologues belong to backward passes only (launch_col_fast)
-  if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true, WIDE>), grid, block, 0, p->stream, a);
-  else if (dir > 0) hipLaunchKernelGGL((col_fft_fast<T, 1, N, false, WIDE>), grid, block, 0, p->stream, a);
-  else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false, WIDE>), grid, block, 0, p->stream, a);
+  if (spectral) hipLaunchKernelGGL((col_fft_fast<T, 1, N, true, WIDE, V>), grid, block, 0, p->stream, a);
+  else if (dir > 0) hipLaunchKernelGGL((col_fft_fast<T, 1, N, false, WIDE, V>), grid, block, 0, p->stream, a);
+  else hipLaunchKernelGGL((col_fft_fast<T, -1, N, false, WIDE, V>), grid, block, 0, p->stream, a);
 }
+
+// radix variant of the size-specialised passes (ddx_plan_set_tuning)
+int radix_variant(const ddx_plan* p) { return p->tune_radix == 1 ? 1 : 0; }
 
 // Size-specialised column kernel for n in {64, 128, 256, 512}
 template <typename T, int N>
@@ -353,7 +356,9 @@ void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bo
   // spectral passes whose caller asks for them
   const bool wide = N == 256 && (p->tune_wide >= 0 ? p->tune_wide == 1
                                                    : (sizeof(T) == 4 || !spectral || a.wide));
-  const int W = wide ? col_width<N, true>() : col_width<N, false>();
+  const int V = radix_variant(p);
+  const int W = V ? (wide ? col_width<N, true, 1>() : col_width<N, false, 1>())
+                  : (wide ? col_width<N, true>() : col_width<N, false>());
   const int ntiles = (a.ncols + W - 1) / W;
   dim3 grid(ntiles, nslabs, nfields);
   a.nslabs = nslabs;
@@ -374,9 +379,14 @@ void col_fast_n(ddx_plan* p, ColArgs<T>& a, int nslabs, int nfields, int dir, bo
     grid = dim3((unsigned)(((units + 7) / 8) * 8 * nfields), 1, 1);
   }
   if constexpr (N == 256) {
-    if (wide) { col_fast_go<T, N, true>(p, a, grid, dir, spectral); return; }
+    if (wide) {
+      if (V) col_fast_go<T, N, true, 1>(p, a, grid, dir, spectral);
+      else col_fast_go<T, N, true, 0>(p, a, grid, dir, spectral);
+      return;
+    }
   }
-  col_fast_go<T, N, false>(p, a, grid, dir, spectral);
+  if (V) col_fast_go<T, N, false, 1>(p, a, grid, dir, spectral);
+  else col_fast_go<T, N, false, 0>(p, a, grid, dir, spectral);
 }
 
 template <typename T>
@@ -494,6 +504,12 @@ void geom_fz(const ddx_plan* p, ColArgs<T>& a) {
 
 template <typename T, int N, int PH, int ND>
 void row_fast_go(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  if (radix_variant(p)) {
+    constexpr int RPB = 64 / RowPlanV<N, 1>::LANES;
+    hipLaunchKernelGGL((row_phys_fast<T, N, PH, ND, 1>), dim3((nrows + RPB - 1) / RPB), dim3(64), 0,
+                       p->stream, a, nrows);
+    return;
+  }
   constexpr int RPB = 64 / RowPlan<N>::LANES;
   hipLaunchKernelGGL((row_phys_fast<T, N, PH, ND>), dim3((nrows + RPB - 1) / RPB), dim3(64), 0,
                      p->stream, a, nrows);
@@ -527,14 +543,20 @@ bool launch_row_fast(ddx_plan* p, const RowArgs<T>& a, long nrows) {
 
 // Size-specialised single-field c2r / r2c row passes (two rows per team);
 // false if none applies.
-template <typename T, int N>
-void row_plain_n(ddx_plan* p, const RowArgs<T>& a, long nrows) {
-  constexpr int RPB = 64 / RowPlan<N>::LANES;
+template <typename T, int N, int V>
+void row_plain_nv(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  constexpr int RPB = 64 / RowPlanV<N, V>::LANES;
   const long teams = (nrows + 1) / 2;
   const int nf = a.mode == ROW_C2R ? a.nin : a.nout;   // grid.y: the field
   const dim3 grid((unsigned)((teams + RPB - 1) / RPB), (unsigned)nf);
-  if (a.mode == ROW_C2R) hipLaunchKernelGGL((row_c2r_fast<T, N>), grid, dim3(64), 0, p->stream, a, nrows);
-  else hipLaunchKernelGGL((row_r2c_fast<T, N>), grid, dim3(64), 0, p->stream, a, nrows);
+  if (a.mode == ROW_C2R) hipLaunchKernelGGL((row_c2r_fast<T, N, V>), grid, dim3(64), 0, p->stream, a, nrows);
+  else hipLaunchKernelGGL((row_r2c_fast<T, N, V>), grid, dim3(64), 0, p->stream, a, nrows);
+}
+
+template <typename T, int N>
+void row_plain_n(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  if (radix_variant(p)) row_plain_nv<T, N, 1>(p, a, nrows);
+  else row_plain_nv<T, N, 0>(p, a, nrows);
 }
 
 template <typename T>
@@ -552,6 +574,26 @@ bool launch_row_plain_fast(ddx_plan* p, const RowArgs<T>& a, long nrows) {
 
 template <typename T>
 int launch_row(ddx_plan* p, RowArgs<T>& a) {
+  const long nrows = (long)p->ay.n * p->az.n;
+  const double cb = sizeof(cpx<T>), rb = sizeof(T);
+  double bytes;
+  int cat;
+  if (a.mode == ROW_PHYS) { cat = PC_ROW_PHYS; bytes = nrows * (double)(a.nin + a.nout) * p->Kx * cb; }
+  else if (a.mode == ROW_C2R) { cat = PC_ROW_C2R; bytes = nrows * (double)a.nin * (p->Kx * cb + a.nx * rb); }
+  else { cat = PC_ROW_R2C; bytes = nrows * (double)a.nout * (p->Kx * cb + a.nx * rb); }
+  // the size-specialised kernels keep rows in registers (any field count)
+  if (!p->generic_only) {
+    ProfScope prof(p, cat, bytes);
+    const bool fast = a.mode == ROW_PHYS ? launch_row_fast<T>(p, a, nrows)
+                                         : launch_row_plain_fast<T>(p, a, nrows);
+    if (fast) {
+      LAUNCHCHK();
+      return 0;
+    }
+    if (prof.a) p->evfree.push_back(prof.a);   // nothing launched: no record
+    prof.a = nullptr;
+  }
+  // the generic kernel stages whole rows of every field in LDS
   const int nseq = std::max((a.nin + 1) / 2, (a.nout + 1) / 2);
   const size_t lds = (size_t)2 * nseq * a.nx * sizeof(cpx<T>);
   if (lds > 65536 && a.mode != ROW_PHYS && std::max(a.nin, a.nout) > 1) {
@@ -567,22 +609,7 @@ int launch_row(ddx_plan* p, RowArgs<T>& a) {
     return 0;
   }
   if (lds > 65536) return fail(DDX_ERR_UNSUPPORTED, "x rows too long for LDS");
-  const long nrows = (long)p->ay.n * p->az.n;
-  const double cb = sizeof(cpx<T>), rb = sizeof(T);
-  double bytes;
-  int cat;
-  if (a.mode == ROW_PHYS) { cat = PC_ROW_PHYS; bytes = nrows * (double)(a.nin + a.nout) * p->Kx * cb; }
-  else if (a.mode == ROW_C2R) { cat = PC_ROW_C2R; bytes = nrows * (double)a.nin * (p->Kx * cb + a.nx * rb); }
-  else { cat = PC_ROW_R2C; bytes = nrows * (double)a.nout * (p->Kx * cb + a.nx * rb); }
   ProfScope prof(p, cat, bytes);
-  if (a.mode == ROW_PHYS && !p->generic_only && launch_row_fast<T>(p, a, nrows)) {
-    LAUNCHCHK();
-    return 0;
-  }
-  if (a.mode != ROW_PHYS && !p->generic_only && launch_row_plain_fast<T>(p, a, nrows)) {
-    LAUNCHCHK();
-    return 0;
-  }
   hipLaunchKernelGGL((row_kernel<T>), dim3(nrows), dim3(256), lds, p->stream, a);
   LAUNCHCHK();
   return 0;
@@ -913,7 +940,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   // runs inside the epilogue; batched plans keep the two launches, which
   // measured as fast for 256 members)
   const bool yepi = !p->zfft && nd == 2 && (p->tune_yepi >= 0 ? p->tune_yepi == 1 : p->nbatch == 1) &&
-                    p->fuse_yepi && !p->generic_only &&
+                    p->fuse_yepi && !p->generic_only && radix_variant(p) == 0 &&
                     p->ay.n == 64 && p->Ky <= 2 * ((64 + 2) / 3) + 1 && L.np <= 3;
   cpx<T>* Uc;
   col_defaults(c);
@@ -1393,7 +1420,7 @@ int ddx_plan_set_graph(ddx_plan* p, int on) {
 int ddx_plan_set_tuning(ddx_plan* p, int knob, int value) {
   if (check_plan(p)) return DDX_ERR_ARG;
   const int top = knob == DDX_TUNE_XCD_ORDER ? 2 : 1;
-  if (knob < DDX_TUNE_WIDE_TILES || knob > DDX_TUNE_FUSED_YEPI || value < -1 || value > top)
+  if (knob < DDX_TUNE_WIDE_TILES || knob > DDX_TUNE_RADIX || value < -1 || value > top)
     return fail(DDX_ERR_ARG, "unknown tuning knob or value");
   if (p->step_exec) {
     HIPCHK(hipStreamSynchronize(p->stream));
@@ -1402,7 +1429,8 @@ int ddx_plan_set_tuning(ddx_plan* p, int knob, int value) {
   }
   if (knob == DDX_TUNE_WIDE_TILES) p->tune_wide = value;
   else if (knob == DDX_TUNE_XCD_ORDER) p->tune_xcd = value;
-  else p->tune_yepi = value;
+  else if (knob == DDX_TUNE_FUSED_YEPI) p->tune_yepi = value;
+  else p->tune_radix = value;
   return 0;
 }
 

@@ -32,14 +32,34 @@ template <> struct ColPlan<128> { static constexpr int LANES = 8,  E = 16, W = 1
 template <> struct ColPlan<256> { static constexpr int LANES = 16, E = 16, W = 8,  NP = 2; static constexpr int R[2] = {16, 16}; };
 template <> struct ColPlan<512> { static constexpr int LANES = 64, E = 8,  W = 4,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
 
+// Radix variants (V = 1, ddx_plan_set_tuning DDX_TUNE_RADIX), for the
+// autotuner to choose: rows of 64..256 points with more points per thread
+// and fewer passes, 512-point rows with radix 4 passes (the FP64 rows of 10
+// and 12 fields keep 8 points per thread, within the registers), and columns
+// with fewer points per thread and more passes.
+template <int N> struct RowPlanAlt;
+template <> struct RowPlanAlt<64>  { static constexpr int LANES = 8,  E = 8,  NP = 2; static constexpr int R[2] = {8, 8}; };
+template <> struct RowPlanAlt<128> { static constexpr int LANES = 16, E = 8,  NP = 3; static constexpr int R[3] = {8, 4, 4}; };
+template <> struct RowPlanAlt<256> { static constexpr int LANES = 32, E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 4}; };
+template <> struct RowPlanAlt<512> { static constexpr int LANES = 64, E = 8,  NP = 4; static constexpr int R[4] = {8, 4, 4, 4}; };
+
+template <int N> struct ColPlanAlt;
+template <> struct ColPlanAlt<64>  { static constexpr int LANES = 16, E = 4,  W = 8, NP = 3; static constexpr int R[3] = {4, 4, 4}; };
+template <> struct ColPlanAlt<128> { static constexpr int LANES = 16, E = 8,  W = 8, NP = 3; static constexpr int R[3] = {8, 4, 4}; };
+template <> struct ColPlanAlt<256> { static constexpr int LANES = 32, E = 8,  W = 4, NP = 3; static constexpr int R[3] = {8, 8, 4}; };
+template <> struct ColPlanAlt<512> { static constexpr int LANES = 32, E = 16, W = 4, NP = 3; static constexpr int R[3] = {16, 8, 4}; };
+
+template <int N, int V> using RowPlanV = typename std::conditional<V == 0, RowPlan<N>, RowPlanAlt<N>>::type;
+template <int N, int V> using ColPlanV = typename std::conditional<V == 0, ColPlan<N>, ColPlanAlt<N>>::type;
+
 // Column tiles of double width (WIDE, n = 256 only): FP32 tiles are then 16
 // columns wide, so a tile row is a whole 128-byte line, and FP64 tiles of
 // the plain passes too (256-byte rows; their exchanges then go through the
 // half-size real buffer): 6 % faster y passes at 256^3.  The FP64 spectral
 // z pass keeps 8 columns, where 16 were slower.
-template <int N, bool WIDE>
+template <int N, bool WIDE, int V = 0>
 constexpr int col_width() {
-  return ColPlan<N>::W * (WIDE ? 2 : 1);
+  return ColPlanV<N, V>::W * (WIDE ? 2 : 1);
 }
 
 // LDS index of position m in the exchange written by a pass of radix R after
@@ -315,9 +335,9 @@ __device__ __forceinline__ void row_loads(cpx<T> (&la)[PL::E], cpx<T> (&lb)[PL::
     }
 }
 
-template <typename T, int N, int PH, int ND>
+template <typename T, int N, int PH, int ND, int V = 0>
 __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nrows) {
-  using PL = RowPlan<N>;
+  using PL = RowPlanV<N, V>;
   constexpr int LANES = PL::LANES, E = PL::E;
   constexpr int RPB = 64 / LANES;
   constexpr int NIN = PhysIO<PH, ND>::NIN, NOUT = PhysIO<PH, ND>::NOUT;
@@ -456,9 +476,9 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
 // c2r of rows 2t and 2t + 1: z = A + i B from the two half spectra, one
 // inverse transform, x rows = (Re z, Im z).  Loads as in row_phys_fast
 // (clamped addresses, zeroed afterwards, the k = 0 imaginary part dropped).
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__(64) row_c2r_fast(const RowArgs<T> a, long nrows) {
-  using PL = RowPlan<N>;
+  using PL = RowPlanV<N, V>;
   constexpr int LANES = PL::LANES, E = PL::E, RPB = 64 / LANES;
   constexpr int R0 = PL::R[0], RL = PL::R[PL::NP - 1];
   constexpr int BUF = exch_size<N, PL>();
@@ -514,9 +534,9 @@ __global__ void __launch_bounds__(64) row_c2r_fast(const RowArgs<T> a, long nrow
 // r2c of rows 2t and 2t + 1: z = a + i b, one forward transform, half
 // spectra A = (Z_k + conj Z_{N-k}) / 2, B = (Z_k - conj Z_{N-k}) / 2i.  The
 // x loads sit at the first-pass positions of the reversed radix order.
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 __global__ void __launch_bounds__(64) row_r2c_fast(const RowArgs<T> a, long nrows) {
-  using PL = RowPlan<N>;
+  using PL = RowPlanV<N, V>;
   constexpr int LANES = PL::LANES, E = PL::E, RPB = 64 / LANES;
   constexpr int RL = PL::R[PL::NP - 1], RF = PL::R[0];
   constexpr int BUF = exch_size<N, PL>();
@@ -661,11 +681,11 @@ __device__ __forceinline__ void col_store(cpx<T>* p, cpx<T> v, bool nt) {
   st_nt(p, v, nt);
 }
 
-template <typename T, int D, int N, bool SPECTRAL, bool WIDE>
-__global__ void __launch_bounds__((ColPlan<N>::LANES * col_width<N, WIDE>()))
+template <typename T, int D, int N, bool SPECTRAL, bool WIDE, int V = 0>
+__global__ void __launch_bounds__((ColPlanV<N, V>::LANES * col_width<N, WIDE, V>()))
 col_fft_fast(const ColArgs<T> a) {
-  using PL = ColPlan<N>;
-  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<N, WIDE>();
+  using PL = ColPlanV<N, V>;
+  constexpr int LANES = PL::LANES, E = PL::E, W = col_width<N, WIDE, V>();
   constexpr int R0 = PL::R[0];
   constexpr int RL = PL::R[PL::NP - 1];
   constexpr int BUF = exch_size<N, PL>();

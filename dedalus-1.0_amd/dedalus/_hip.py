@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libddx.so")
 F64, F32 = 0, 1
 HYDRO, BOUSSINESQ, MHD = 0, 1, 2
 RK2MID, RK2TRAP = 0, 1
-TUNE_WIDE_TILES, TUNE_XCD_ORDER, TUNE_FUSED_YEPI = 0, 1, 2
+TUNE_WIDE_TILES, TUNE_XCD_ORDER, TUNE_FUSED_YEPI, TUNE_RADIX = 0, 1, 2, 3
 EULER, ETD1, ETD2RK1, ETD2RK2 = 0, 1, 2, 3
 
 ERR_NAMES = {-1: "DDX_ERR_ARG", -2: "DDX_ERR_HIP", -3: "DDX_ERR_UNSUPPORTED",
@@ -276,8 +276,9 @@ class Plan:
         _check(_lib.ddx_plan_set_graph(self.handle, 1 if on else 0))
 
     def set_tuning(self, knob, value):
-        """A kernel choice (TUNE_*) that leaves results bit-identical (FP64
-        tile widths: equal to rounding); -1 restores the built-in rule."""
+        """A kernel choice (TUNE_*): bit-identical results, except FP64 tile
+        widths and the radix plan (TUNE_RADIX), equal to rounding; -1
+        restores the built-in rule."""
         _check(_lib.ddx_plan_set_tuning(self.handle, int(knob), int(value)))
 
     def set_stream(self, stream):

@@ -165,13 +165,14 @@ class Ensemble:
 
     def autotune(self, steps=4, reps=2):
         """Choose, per group, the kernel variants that run its step fastest:
-        16-column tiles (256-point groups), the XCD order of the spectral y
-        pass, and the fused forward y pass and epilogue (64-point groups).
-        Every variant gives bit-identical results in FP32 (FP64 tile widths
-        agree to rounding).  Each candidate is timed
-        with HIP events over `steps` eager steps, best of `reps`, and the
-        members' state is restored afterwards.  Returns {shape: choice};
-        capture() afterwards to replay the tuned kernels."""
+        the radix plan of the row and column transforms, 16-column tiles
+        (256-point groups), the XCD order of the spectral y pass, and the
+        fused forward y pass and epilogue (64-point groups, built-in radix).
+        The layout choices give bit-identical results in FP32 (FP64 tile
+        widths agree to rounding); the radix plan agrees to rounding.  Each
+        candidate is timed with HIP events over `steps` eager steps, best of
+        `reps`, and the members' state is restored afterwards.  Returns
+        {shape: choice}; capture() afterwards to replay the tuned kernels."""
         import itertools
         import torch
         self.graph = None
@@ -184,9 +185,12 @@ class Ensemble:
                     g.plan.copy(b, s)
                 wide = (0, 1) if g.shape[0] == 256 else (-1,)
                 yepi = (0, 1) if g.shape[0] == 64 else (-1,)
+                knobs = (_hip.TUNE_WIDE_TILES, _hip.TUNE_XCD_ORDER, _hip.TUNE_FUSED_YEPI, _hip.TUNE_RADIX)
                 best = None
-                for cand in itertools.product(wide, (0, 1, 2), yepi):
-                    for knob, v in zip((_hip.TUNE_WIDE_TILES, _hip.TUNE_XCD_ORDER, _hip.TUNE_FUSED_YEPI), cand):
+                for cand in itertools.product(wide, (0, 1, 2), yepi, (0, 1)):
+                    if cand[3] == 1 and cand[2] == 1:
+                        continue                      # radix 1 runs the y pass and epilogue apart
+                    for knob, v in zip(knobs, cand):
                         g.plan.set_tuning(knob, v)
                     g.step(self.params, 1, self.dt)          # first use of the variant
                     t = None
@@ -200,13 +204,13 @@ class Ensemble:
                         t = ms if t is None else min(t, ms)
                     if best is None or t < best[0]:
                         best = (t, cand)
-                for knob, v in zip((_hip.TUNE_WIDE_TILES, _hip.TUNE_XCD_ORDER, _hip.TUNE_FUSED_YEPI), best[1]):
+                for knob, v in zip(knobs, best[1]):
                     g.plan.set_tuning(knob, v)
                 for b, s in zip(saved, g.state):
                     g.plan.copy(s, b)
                 self.stream.synchronize()
                 chosen[g.shape] = dict(wide_tiles=best[1][0], xcd_order=best[1][1], fused_yepi=best[1][2],
-                                       ms_per_step=best[0])
+                                       radix=best[1][3], ms_per_step=best[0])
         return chosen
 
     def advance(self, nsteps):

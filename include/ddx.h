@@ -118,17 +118,25 @@ int ddx_plan_set_graph(ddx_plan* plan, int on);
 /* Kernel choices of a plan that do not change its results, for autotuning:
  * every choice is bit-identical, except that FP64 16-column tiles exchange
  * through a buffer of reals (other instruction sequences) and agree with
- * 8-column ones to rounding.  value -1 restores the built-in rule.
+ * 8-column ones to rounding, and the radix variant (below) agrees to
+ * rounding.  value -1 restores the built-in rule.
  *   DDX_TUNE_WIDE_TILES: 16-column tiles of the 256-point column passes
  *     (0 off, 1 on);
  *   DDX_TUNE_XCD_ORDER: workgroup order of the spectral column passes over
  *     the 8 XCDs (0 plain grid, 1 the fields of a (slab, tile) together,
  *     2 whole slabs together);
  *   DDX_TUNE_FUSED_YEPI: forward y pass inside the epilogue for 64-point
- *     y axes of 2D plans, batched ones included (0 off, 1 on).
+ *     y axes of 2D plans, batched ones included (0 off, 1 on);
+ *   DDX_TUNE_RADIX: radix plan of the size-specialised row and column
+ *     transforms (0 the built-in plans; 1 rows with more points per thread
+ *     and fewer passes, e.g. 256 = 8.8.4 instead of 4.4.4.4, and columns
+ *     with fewer points per thread and more passes, e.g. 256 = 8.8.4
+ *     instead of 16.16).  Results agree with the built-in plans to rounding
+ *     (other butterflies), not bit for bit; radix 1 runs the forward y pass
+ *     and the epilogue as two launches.
  * The step graph is recaptured at the next ddx_step.  No reference
  * counterpart: the choices belong to this implementation. */
-enum { DDX_TUNE_WIDE_TILES = 0, DDX_TUNE_XCD_ORDER = 1, DDX_TUNE_FUSED_YEPI = 2 };
+enum { DDX_TUNE_WIDE_TILES = 0, DDX_TUNE_XCD_ORDER = 1, DDX_TUNE_FUSED_YEPI = 2, DDX_TUNE_RADIX = 3 };
 int ddx_plan_set_tuning(ddx_plan* plan, int knob, int value);
 
 /* ---- memory --------------------------------------------------------------- */

@@ -191,9 +191,11 @@ def test_member_reductions(precision):
 
 
 def test_autotune_keeps_results():
-    """Ensemble.autotune picks kernel variants per group (tile width, XCD
-    order, fused forward y + epilogue); it restores the members' state, and
-    the tuned graph's steps equal the untuned ones bit for bit."""
+    """Ensemble.autotune picks kernel variants per group (radix plan, tile
+    width, XCD order, fused forward y + epilogue); it restores the members'
+    state bit for bit, and the tuned graph's steps equal the untuned ones
+    (bit for bit with the built-in radix plan, to FP32 rounding with the
+    other)."""
     hip()
     from dedalus.ensemble import Ensemble
     case("ensemble autotune")
@@ -215,9 +217,13 @@ def test_autotune_keeps_results():
         ens.advance(4)
         ens.sync()
         runs.append([g.read() for g in ens.groups])
-    for ga, gb in zip(*runs):
+    for ga, gb, shape in zip(runs[0], runs[1], [g[0] for g in groups]):
+        exact = chosen[shape]["radix"] == 0
         for (ua, va), (ub, vb) in zip(ga, gb):
-            assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+            if exact:
+                assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+            else:
+                assert rel_l2(ua, ub) < 1e-5 and rel_l2(va, vb) < 1e-5
 
 
 def test_tuning_knobs_keep_results():

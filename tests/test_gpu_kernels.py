@@ -591,3 +591,80 @@ def test_fused_update_matches_separate_sweeps(kind, shape, integ, precision):
         out.append([plan.download_k(b) for b in st])
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+RADIX_SHAPES = [(64, 64), (128, 128), (256, 256), (512, 512), (16, 64, 128), (8, 256, 64),
+                (512, 16, 64), (4, 8, 512)]
+
+
+@pytest.mark.parametrize("shape", RADIX_SHAPES, ids=str)
+def test_radix_variant_transforms(shape):
+    """The radix variant of the size-specialised rows and columns
+    (DDX_TUNE_RADIX = 1: rows 8.8 / 8.4.4 / 8.8.4 / 8.4.4.4, columns 4.4.4 /
+    8.4.4 / 8.8.4 / 16.8.4) transforms like the oracle."""
+    hp = hip()
+    L = LENGTHS[len(shape)]
+    case("radix variant transforms", shape)
+    plan = hp.Plan(shape, L)
+    plan.set_tuning(hp.TUNE_RADIX, 1)
+    comp = npth.Component(shape, L)
+    x = random_x(shape, 31)
+    comp['xspace'] = x
+    ref = comp['kspace']
+    xd, kc = plan.alloc_x(), plan.alloc_k()
+    plan.upload_x(xd, x)
+    plan.fwd(xd, kc)
+    assert rel_l2(plan.download_k(kc), ref) < 1e-14
+    k = random_k(shape, 32, L)
+    comp['kspace'] = k
+    xr = comp['xspace']
+    plan.upload_k(kc, k)
+    plan.bwd(kc, xd)
+    assert rel_l2(plan.download_x(xd), xr) < 1e-14
+
+
+@pytest.mark.parametrize("kind,shape", [('hydro', (64, 64)), ('hydro', (128, 128)), ('boussinesq', (256, 64)),
+                                        ('mhd', (16, 64, 128)), ('mhd', (8, 64, 512)), ('hydro', (512, 16, 64))],
+                         ids=str)
+def test_radix_variant_steps(kind, shape):
+    """Three RK2mid steps with the radix variant: the oracle to 1e-12, the
+    built-in plans to rounding."""
+    hp = hip()
+    nd = len(shape)
+    L = (2 * np.pi,) * nd
+    case("radix variant steps", kind, shape)
+    p = {'nu': 1e-3, 'eta': 1e-3, 'kappa': 1e-3}
+    if kind == 'boussinesq':
+        p['boussinesq_direction'] = 'y' if nd == 2 else 'z'
+    kf = kfields(kind, shape, 33, L)
+    ref = npth.run(kind, shape, kf, 2e-3, 3, 'RK2mid', L, **dict(p))
+    prm = params_struct(hp, kind, nd, p)
+    got = []
+    for radix in (1, 0):
+        plan = hp.Plan(shape, L)
+        plan.set_tuning(hp.TUNE_RADIX, radix)
+        st = upload_fields(plan, kind, kf)
+        bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+        plan.step(hp.RK2MID, prm, st, bufs[0], bufs[1], None, 3, 2e-3)
+        got.append(download_fields(plan, kind, nd, st))
+    for name in FIELDS[kind]:
+        for c in range(len(ref[name])):
+            assert rel_l2(got[0][name][c], ref[name][c]) < 1e-12, (name, c)
+            assert rel_l2(got[0][name][c], got[1][name][c]) < 1e-13, (name, c)
+
+
+def test_radix_variant_fp32():
+    """FP32 with the radix variant against the FP64 oracle (128^2 hydro)."""
+    hp = hip()
+    shape = (128, 128)
+    L = (2 * np.pi,) * 2
+    case("radix variant fp32", shape)
+    kf = kfields('hydro', shape, 34, L)
+    ref = npth.run('hydro', shape, kf, 2e-3, 3, 'RK2mid', L, nu=1e-3)
+    plan = hp.Plan(shape, L, hp.F32)
+    plan.set_tuning(hp.TUNE_RADIX, 1)
+    st = upload_fields(plan, 'hydro', kf)
+    bufs = [[plan.alloc_k() for _ in st] for _ in range(3)]
+    plan.step(hp.RK2MID, params_struct(hp, 'hydro', 2, {'nu': 1e-3}), st, bufs[0], bufs[1], None, 3, 2e-3)
+    for c in range(2):
+        assert rel_l2(plan.download_k(st[c]), ref['u'][c]) < 2e-6
