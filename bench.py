@@ -82,11 +82,28 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def gpu_run(w, steps, warmup, precision="f64"):
+TUNE_KNOBS = {"wide": "TUNE_WIDE_TILES", "xcd": "TUNE_XCD_ORDER", "yepi": "TUNE_FUSED_YEPI",
+              "radix": "TUNE_RADIX"}
+
+
+def parse_tune(text):
+    """'radix=1,wide=0' -> {'radix': 1, 'wide': 0} (kernel choices, ddx_plan_set_tuning)."""
+    out = {}
+    for item in filter(None, (text or "").split(",")):
+        k, _, v = item.partition("=")
+        if k not in TUNE_KNOBS:
+            raise SystemExit("--tune: unknown knob %r (one of %s)" % (k, ", ".join(TUNE_KNOBS)))
+        out[k] = int(v)
+    return out
+
+
+def gpu_run(w, steps, warmup, precision="f64", tune=None):
     from dedalus import _hip
 
     nd = len(w["shape"])
     plan = _hip.Plan(w["shape"], w["length"], _hip.F64 if precision == "f64" else _hip.F32)
+    for k, v in (tune or {}).items():
+        plan.set_tuning(getattr(_hip, TUNE_KNOBS[k]), v)
     names = {"hydro": ["u"], "boussinesq": ["u", "T"], "mhd": ["u", "B"]}[w["kind"]]
     state = []
     for nm in names:
@@ -487,6 +504,9 @@ def main():
     ap.add_argument("--workload", default="mhd3d_256")
     ap.add_argument("--precision", choices=["f64", "f32"], default="f64",
                     help="f32: FP32 transforms and products (FP64 twiddles, projection, ETD)")
+    ap.add_argument("--tune", default="",
+                    help="kernel choices for the plan, e.g. radix=1,wide=1 (knobs: %s; "
+                         "default: the built-in rule)" % ", ".join(TUNE_KNOBS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-closure", action="store_true",
                     help="skip the learned-closure measurement (the metric's second part)")
@@ -518,7 +538,7 @@ def main():
         npts = 1
         for n in w["shape"]:
             npts *= n
-        wall, prof = gpu_run(w, args.steps, args.warmup, args.precision)
+        wall, prof = gpu_run(w, args.steps, args.warmup, args.precision, parse_tune(args.tune))
     wall_instr = prof.pop("_wall_instrumented_s")
     ens_info = prof.pop("_ensemble", None)
     if args.calibrate_copy_mib:
@@ -589,6 +609,8 @@ def main():
         "kernels": kernels,
         "source_hash": source_hash(),
     }
+    if args.tune:
+        out["config"]["tuning"] = parse_tune(args.tune)
     if ens_info:
         out["ensemble"] = ens_info
     if not args.no_closure and w["name"] == "mhd3d_256":

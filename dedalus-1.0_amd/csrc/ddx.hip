@@ -1003,7 +1003,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
 template <typename T>
 int update_impl(ddx_plan* p, int kind, int ncomp, const cpx<T>* const* y, cpx<T>* const* out,
                 const cpx<T>* const* F1, const cpx<T>* const* F2, const double* coeff, int order,
-                double dt) {
+                double dt, const double* ifac = nullptr) {
   if (ncomp < 1 || ncomp > DDX_MAXF) return fail(DDX_ERR_ARG, "bad component count");
   if (kind < DDX_EULER || kind > DDX_ETD2RK2) return fail(DDX_ERR_ARG, "bad update kind");
   UpdArgs<T> u;
@@ -1017,6 +1017,7 @@ int update_impl(ddx_plan* p, int kind, int ncomp, const cpx<T>* const* y, cpx<T>
     u.F2[i] = F2 ? F2[i] : nullptr;
     u.coeff[i] = coeff[i];
   }
+  u.ifac = ifac;
   u.g = kgeom(p);
   u.nk = p->nk;
   u.ncomp = ncomp;
@@ -1026,7 +1027,8 @@ int update_impl(ddx_plan* p, int kind, int ncomp, const cpx<T>* const* y, cpx<T>
   u.dt = dt;
   const int bs = 256;
   const double kvalid = (double)p->Ky * p->Kz * p->Kx;
-  ProfScope prof(p, PC_UPDATE, kvalid * ncomp * (kind >= DDX_ETD2RK1 ? 4 : 3) * sizeof(cpx<T>));
+  ProfScope prof(p, PC_UPDATE, kvalid * (ncomp * (kind >= DDX_ETD2RK1 ? 4 : 3) * sizeof(cpx<T>) +
+                                         (ifac ? sizeof(double) : 0)));
   hipLaunchKernelGGL((update_kernel<T>), dim3((p->nk + bs - 1) / bs), dim3(bs), 0, p->stream, u);
   LAUNCHCHK();
   return 0;
@@ -1623,6 +1625,25 @@ int ddx_update(ddx_plan* p, int kind, const void* y, void* out, const void* F1, 
   const cpx<float>* f1 = (const cpx<float>*)F1;
   const cpx<float>* f2 = (const cpx<float>*)F2;
   return update_impl<float>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, order, dt);
+}
+
+int ddx_update_if(ddx_plan* p, int kind, const void* y, void* out, const void* F1, const void* F2,
+                  const double* intfactor, double dt) {
+  if (check_plan(p)) return DDX_ERR_ARG;
+  if (!intfactor) return fail(DDX_ERR_ARG, "null integrating factor");
+  const double coeff = 0.0;
+  if (p->precision == DDX_F64) {
+    const cpx<double>* yy = (const cpx<double>*)y;
+    cpx<double>* oo = (cpx<double>*)out;
+    const cpx<double>* f1 = (const cpx<double>*)F1;
+    const cpx<double>* f2 = (const cpx<double>*)F2;
+    return update_impl<double>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, 1, dt, intfactor);
+  }
+  const cpx<float>* yy = (const cpx<float>*)y;
+  cpx<float>* oo = (cpx<float>*)out;
+  const cpx<float>* f1 = (const cpx<float>*)F1;
+  const cpx<float>* f2 = (const cpx<float>*)F2;
+  return update_impl<float>(p, kind, 1, &yy, &oo, &f1, &f2, &coeff, 1, dt, intfactor);
 }
 
 int ddx_rhs(ddx_plan* p, const ddx_params* prm, const void* const* state, void* const* deriv,

@@ -57,18 +57,21 @@ __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h <<
 
 // GELU in its tanh form, written as z sigma(s) with
 // s = 2 sqrt(2/pi) (z + 0.044715 z^3): 0.5 z (1 + tanh(s/2)) = z sigma(s), so
-// the negative tail has no cancellation.  One hardware exp and one
-// reciprocal per element; sigma(s) = 1 / (1 + e^-s) saturates to 0 and 1.
+// the negative tail has no cancellation.  sigma(s) = 1 / (1 + 2^u) with
+// u = -s log2(e) = z (A + B z^2): the constants fold -2 sqrt(2/pi) log2(e),
+// so one multiply, one fma and one multiply form the exponent of the
+// hardware exp2, and one reciprocal finishes sigma (saturating to 0 and 1).
 //   GELU'(z) = sigma(s) + z sigma(s) (1 - sigma(s)) s'(z),
 //   s'(z) = 2 sqrt(2/pi) (1 + 3 * 0.044715 z^2)
-__device__ inline float sigmoid_fast(float s) { return __builtin_amdgcn_rcpf(1.0f + __expf(-s)); }
-__device__ inline float gelu(float z) {
-  const float s = 1.5957691216057308f * (z + 0.044715f * z * z * z);
-  return z * sigmoid_fast(s);
+constexpr float GELU_A = -2.3022082f;            // -2 sqrt(2/pi) log2(e)
+constexpr float GELU_B = -2.3022082f * 0.044715f;
+__device__ inline float gelu_sigma(float z) {
+  const float u = z * __builtin_fmaf(z * z, GELU_B, GELU_A);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u));
 }
+__device__ inline float gelu(float z) { return z * gelu_sigma(z); }
 __device__ inline float gelu_grad(float z) {
-  const float s = 1.5957691216057308f * (z + 0.044715f * z * z * z);
-  const float sg = sigmoid_fast(s);
+  const float sg = gelu_sigma(z);
   const float ds = 1.5957691216057308f * (1.0f + 0.134145f * z * z);
   return sg + z * sg * (1.0f - sg) * ds;
 }
@@ -929,20 +932,23 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* w_l,
     for (int i = 0; i < 4; ++i) bv[j][i] = b_l[j * 16 + 4 * kq + i];
   const int npx = w * h, nfrag = (npx + 15) / 16;
   const int wi = w + 2;
-  for (int f0 = 2 * wave; f0 < nfrag; f0 += 2 * NW) {
-    // two fragments per pass: four independent accumulation chains per wave
-    int pp[2], base[2];
-    bool valid[2];
+  const int npairs = (nfrag + 1) / 2;
+  // A pair of 16-pixel fragments per step: four independent accumulation
+  // chains per wave.  The steps are software-pipelined over two accumulator
+  // sets: the MFMAs of one pair are issued, then the epilogue (bias, GELU,
+  // bf16 stores: VALU work) of the previous pair runs while the matrix cores
+  // work through them.
+  auto compute = [&](int pr, f32x4 (&acc)[2][NCO], int (&pp)[2], bool (&valid)[2]) {
+    int base[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      int p = (f0 + u) * 16 + m;
+      int p = (2 * pr + u) * 16 + m;
       valid[u] = p < npx;
       if (!valid[u]) p = npx - 1;
       const int r = p / w, c = p - r * w;
       pp[u] = p;
       base[u] = r * wi + c;
     }
-    f32x4 acc[2][NCO];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -966,6 +972,8 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* w_l,
         }
       }
     }
+  };
+  auto finish = [&](const f32x4 (&acc)[2][NCO], const int (&pp)[2], const bool (&valid)[2]) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       if (!LAST) {
@@ -1002,7 +1010,26 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* w_l,
         }
       }
     }
+  };
+  f32x4 accA[2][NCO], accB[2][NCO];
+  int ppA[2], ppB[2];
+  bool vA[2], vB[2];
+  bool pendA = false, pendB = false;
+  for (int pr = wave; pr < npairs;) {
+    compute(pr, accA, ppA, vA);
+    if (pendB) finish(accB, ppB, vB);
+    pendA = true;
+    pendB = false;
+    pr += NW;
+    if (pr >= npairs) break;
+    compute(pr, accB, ppB, vB);
+    finish(accA, ppA, vA);
+    pendA = false;
+    pendB = true;
+    pr += NW;
   }
+  if (pendA) finish(accA, ppA, vA);
+  if (pendB) finish(accB, ppB, vB);
 }
 
 template <int L, int NL, int HID, int TX, int TY, int NT>

@@ -376,6 +376,8 @@ struct UpdArgs {
   const cpx<T>* F1[DDX_MAXF];
   const cpx<T>* F2[DDX_MAXF];
   double coeff[DDX_MAXF];
+  // per-mode integrating factor (compact layout, FP64), or null: coeff |k|^(2 order)
+  const double* ifac;
   KGeom g;
   long nk;
   int ncomp, kind, order, ndim;
@@ -420,12 +422,19 @@ __global__ void update_kernel(const UpdArgs<T> a) {
   else kp = pow(k2, (double)a.order);
   double cprev = 0.0, Z = 0.0, p0 = 1.0, p1 = 1.0, p2 = 0.5;
   bool have = false;
+  if (a.ifac) {
+    // an integrating-factor array: the reference passes -IF to the sweep,
+    // which forms Z = (-IF) dt (forward_step_cy_3d.pyx:54)
+    Z = a.kind != UPD_EULER ? (-a.ifac[idx]) * dt : 0.0;
+    if (Z != 0.0) phi012(Z, a.ndim == 2, p0, p1, p2);
+    have = true;
+  }
   for (int comp = 0; comp < a.ncomp; ++comp) {
     const cpx<double> y = c64(a.y[comp][idx]);
     const cpx<double> f1 = c64(a.F1[comp][idx]);
     const cpx<double> f2 = (a.kind >= UPD_ETD2RK1) ? c64(a.F2[comp][idx]) : mk(0.0, 0.0);
     const double c = a.coeff[comp];
-    if (!have || c != cprev) {
+    if (!a.ifac && (!have || c != cprev)) {
       Z = (c != 0.0 && a.kind != UPD_EULER) ? (-(c * kp)) * dt : 0.0;
       if (Z != 0.0) phi012(Z, a.ndim == 2, p0, p1, p2);
       cprev = c;

@@ -89,6 +89,7 @@ SIGNATURES = {
     "ddx_deriv": (_I, [_VP, _VP, _VP, _I]),
     "ddx_project": (_I, [_VP, _PP]),
     "ddx_update": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _D, _I, _D]),
+    "ddx_update_if": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _D]),
     "ddx_rhs": (_I, [_VP, ctypes.POINTER(Params), _PP, _PP, _PP]),
     "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
     "ddx_step_noalloc": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
@@ -170,6 +171,32 @@ def ptr_array(ptrs):
     for i, p in enumerate(ptrs):
         arr[i] = p
     return arr
+
+
+def compact_real(ref, kref_shape, kc_shape, cut):
+    """Reference-layout real k array -> compact float64 array of
+    Ky * Kz * Kxp entries (the modes ddx_pack keeps, in its order; zero on
+    the padding columns kx index > cut[0]).  Reference layouts: (kx, ky) in
+    2D, (ky, kz, kx) in 3D (representations.py:204-233)."""
+    ref = np.asarray(ref, dtype=np.float64)
+    if ref.shape != tuple(kref_shape):
+        raise ValueError("k array shape %s, expected %s" % (ref.shape, tuple(kref_shape)))
+    Ky, Kz, Kxp = kc_shape
+    cx, cy, cz = cut
+    ndim = len(kref_shape)
+    ny = kref_shape[1] if ndim == 2 else kref_shape[0]
+    slab = np.arange(Ky)
+    fy = np.where(slab <= cy, slab, slab + (ny - Ky))
+    col = np.arange(min(Kxp, cx + 1))
+    out = np.zeros((Ky, Kz, Kxp))
+    if ndim == 2:
+        out[:, 0, :col.size] = ref[np.ix_(col, fy)].T
+    else:
+        nz = kref_shape[1]
+        row = np.arange(Kz)
+        fz = np.where(row <= cz, row, row + (nz - Kz))
+        out[:, :, :col.size] = ref[np.ix_(fy, fz, col)]
+    return out.reshape(-1)
 
 
 class DevBuf:
@@ -390,6 +417,22 @@ class Plan:
         _check(_lib.ddx_update(self.handle, kind, int(y), int(out), int(F1),
                                int(F2) if F2 is not None else None,
                                float(coeff), int(order), float(dt)))
+
+    def update_if(self, kind, y, out, F1, F2, intfactor, dt):
+        """The sweep of ``update`` with a per-mode integrating factor: a
+        device buffer of nkc float64 values in the compact layout
+        (``compact_real`` of the reference-layout array)."""
+        _check(_lib.ddx_update_if(self.handle, kind, int(y), int(out), int(F1),
+                                  int(F2) if F2 is not None else None, int(intfactor), float(dt)))
+
+    def upload_real_k(self, ref, buf=None):
+        """Reference-layout real k array (host) -> compact float64 device
+        buffer (a new one unless ``buf`` is given)."""
+        host = compact_real(ref, self.kref_shape, self.kc_shape, self.cut)
+        if buf is None:
+            buf = DevBuf(host.nbytes)
+        self.h2d(buf, host)
+        return buf
 
     def kcombine(self, out, terms, accumulate=False):
         """out (+)= sum coef * M(op) * src over (src, coef, op) terms (op -1, 0..2, 3)."""

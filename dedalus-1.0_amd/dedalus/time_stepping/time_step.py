@@ -145,14 +145,32 @@ class TimeStepBase:
         return dt
 
 
-def _if_args(IF):
-    """(coeff, order) of an integrating factor for the device, None for None."""
+def _if_args(IF, plan):
+    """How a sweep applies an integrating factor on the device: None for
+    None, ('coeff', coeff, order) for coeff*|k|^(2 order), which the kernels
+    recompute from the wavenumbers, and ('array', buffer) for any other array
+    (a per-mode factor in the reference k layout, uploaded in compact form)."""
     if IF is None:
         return None
     if isinstance(IF, IntegratingFactor):
-        return IF.coeff, IF.order
-    raise NotImplementedError("integrating factors other than coeff*|k|^(2 order) "
-                              "are not part of the MI355X path")
+        return ('coeff', IF.coeff, IF.order)
+    return ('array', plan.upload_real_k(np.asarray(IF, dtype=np.float64)))
+
+
+def _sweep(plan, kind, IF, y, out, F1, F2, dt):
+    """One ETD sweep of a component with its integrating factor (kind is
+    the ETD form; Euler when IF is None, as time_step.py:290-292 do)."""
+    a = _if_args(IF, plan)
+    if a is None:
+        if kind == _hip.ETD2RK1:
+            plan.update(_hip.ETD2RK1, y, out, F1, F2, 0., 1, dt)
+        else:
+            plan.update(_hip.EULER, y, out, F2 if kind == _hip.ETD2RK2 else F1, None, 0., 1, dt)
+    elif a[0] == 'coeff':
+        plan.update(kind, y, out, F1, F2, a[1], a[2], dt)
+    else:
+        plan.update_if(kind, y, out, F1, F2, a[1], dt)
+        plan.sync()  # the uploaded factor is freed on return
 
 
 class RKBase(TimeStepBase):
@@ -161,9 +179,12 @@ class RKBase(TimeStepBase):
     def _fusable(self):
         ph = self.RHS
         # exactly a built-in class: a subclass may rely on the reference's
-        # class-dependent projection (physics.py:575, 697, 805)
+        # class-dependent projection (physics.py:575, 697, 805); and every
+        # integrating factor of the form the fused step recomputes
         return (type(ph) in FUSED_CLASSES and not ph.forcing_functions
-                and ph._phys_id is not None)
+                and ph._phys_id is not None
+                and all(isinstance(getattr(c, 'integrating_factor', None), (IntegratingFactor, type(None)))
+                        for c in self._comps(self.deriv1)))
 
     def _comps(self, sd):
         return self.RHS._state_comps(sd)
@@ -222,23 +243,15 @@ class RK2mid(RKBase):
     def _sweep_stage1(self, data, dt):
         plan = self._plan(data)
         for y, a2, f1 in zip(self._comps(data), self._comps(self.data2), self._comps(self.deriv1)):
-            a = _if_args(f1.integrating_factor)
-            yb, fb, ob = y._kdev(), f1._kdev(), a2._kout()
-            if a is None:
-                plan.update(_hip.EULER, yb, ob, fb, None, 0., 1, dt / 2.)
-            else:
-                plan.update(_hip.ETD1, yb, ob, fb, None, a[0], a[1], dt / 2.)
+            _sweep(plan, _hip.ETD1, f1.integrating_factor, y._kdev(), a2._kout(), f1._kdev(), None,
+                   dt / 2.)
             a2._mark('kspace')
 
     def _sweep_stage2(self, data, dt):
         plan = self._plan(data)
         for y, f1, f2 in zip(self._comps(data), self._comps(self.deriv1), self._comps(self.deriv2)):
-            a = _if_args(f1.integrating_factor)
-            yb, f1b, f2b = y._kdev(), f1._kdev(), f2._kdev()
-            if a is None:
-                plan.update(_hip.EULER, yb, yb, f2b, None, 0., 1, dt)
-            else:
-                plan.update(_hip.ETD2RK2, yb, yb, f1b, f2b, a[0], a[1], dt)
+            yb = y._kdev()
+            _sweep(plan, _hip.ETD2RK2, f1.integrating_factor, yb, yb, f1._kdev(), f2._kdev(), dt)
             y._mark('kspace')
 
 
@@ -270,22 +283,16 @@ class RK2trap(RKBase):
             plan = self._plan(data)
             self.RHS.RHS(data, self.deriv1)
             for y, f1 in zip(self._comps(data), self._comps(self.deriv1)):
-                a = _if_args(f1.integrating_factor)
-                yb, fb = y._kdev(), f1._kdev()
-                if a is None:
-                    plan.update(_hip.EULER, yb, yb, fb, None, 0., 1, dt)
-                else:
-                    plan.update(_hip.ETD1, yb, yb, fb, None, a[0], a[1], dt)
+                yb = y._kdev()
+                _sweep(plan, _hip.ETD1, f1.integrating_factor, yb, yb, f1._kdev(), None, dt)
                 y._mark('kspace')
             data.set_time(data.time + dt)
             self.RHS.RHS(data, self.deriv2)
             for y, f1, f2 in zip(self._comps(data), self._comps(self.deriv1),
                                  self._comps(self.deriv2)):
-                a = _if_args(f1.integrating_factor)
-                yb, f1b, f2b = y._kdev(), f1._kdev(), f2._kdev()
+                yb = y._kdev()
                 # Euler on F2 - F1 with dt/2 is ETD2RK1 at Z = 0
-                plan.update(_hip.ETD2RK1, yb, yb, f1b, f2b,
-                            0. if a is None else a[0], 1 if a is None else a[1], dt)
+                _sweep(plan, _hip.ETD2RK1, f1.integrating_factor, yb, yb, f1._kdev(), f2._kdev(), dt)
                 y._mark('kspace')
         self.time += dt
         self.iteration += 1

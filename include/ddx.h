@@ -194,6 +194,14 @@ int ddx_project(ddx_plan* plan, void* const* comps);
  * 3D plans exp() as forward_step_cy_3d.pyx:54-56. */
 int ddx_update(ddx_plan* plan, int kind, const void* y, void* out, const void* F1,
                const void* F2, double coeff, int order, double dt);
+/* The same sweep with an integrating factor given per mode: intfactor is a
+ * device array of FP64 values in the compact k layout (the component's
+ * integrating_factor array, positive for dissipation), and Z = (-IF) dt as
+ * the reference's sweeps form it from the negated array they are passed
+ * (time_step.py:289-304, 373-388; forward_step_cy_3d.pyx:34-60).  For
+ * integrating factors that are not coeff*|k|^(2*order). */
+int ddx_update_if(ddx_plan* plan, int kind, const void* y, void* out, const void* F1,
+                  const void* F2, const double* intfactor, double dt);
 
 /* Linear combination on compact k arrays (the k-space operators of
  * physics.py:160-195, 266-307, 374-416 and fields.py:169-195):

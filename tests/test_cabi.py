@@ -5,6 +5,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -49,3 +50,38 @@ def test_errors_without_init():
         rc = lib.ddx_plan_create(2, sh, ln, 0, ctypes.byref(p))
         assert rc == -5
         assert b"ddx_init" in lib.ddx_last_error()
+
+
+@pytest.mark.parametrize("xshape,length", [((12, 18), (1.0, 2.0)), ((10, 12, 16), (1.0, 1.5, 2.0))],
+                         ids=str)
+def test_compact_real_layout(xshape, length):
+    """compact_real (the host gather behind ddx_update_if's per-mode
+    integrating factors) keeps exactly the modes the 2/3 rule keeps, in the
+    compact order ddx_pack uses: gathering the reference |k|^2 array gives
+    |k|^2 of the compact wavenumber vectors, and zeros on the pad columns."""
+    from dedalus import _hip
+    from dedalus.data_objects import kgrid
+    k, kny, ksh = kgrid.wavenumbers(xshape, length)
+    kt = kgrid.ktrans(len(xshape))
+    mask = kgrid.dealias_mask(xshape, length)
+    ref = kgrid.k2(xshape, length)
+
+    def kept(name):
+        kk = k[name].reshape(-1)
+        return kk[np.abs(kk) < 2. / 3. * kny[kt[name]]]
+
+    kx, ky = kept('x'), kept('y')
+    kz = kept('z') if len(xshape) == 3 else np.zeros(1)
+    pad = 3
+    kc_shape = (ky.size, kz.size, kx.size + pad)
+    cut = (kx.size - 1, (ky.size - 1) // 2, (kz.size - 1) // 2)
+    kref_shape = ksh
+    out = _hip.compact_real(ref, kref_shape, kc_shape, cut).reshape(kc_shape)
+    expect = (ky[:, None, None] ** 2 + kz[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    assert np.array_equal(out[:, :, :kx.size], expect)
+    assert not out[:, :, kx.size:].any()
+    # every kept reference entry is gathered exactly once
+    tag = np.arange(ref.size, dtype=np.float64).reshape(ref.shape) + 1.0
+    got = _hip.compact_real(tag, kref_shape, kc_shape, cut)
+    got = np.sort(got[got > 0])
+    assert np.array_equal(got, np.sort(tag[~mask]))

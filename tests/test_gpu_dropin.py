@@ -293,3 +293,91 @@ def test_subclass_runs_reference_operator_sequence(kind, shape, p):
         ti.advance(data, 5e-3)
         oti.do_advance(odata, 5e-3)
     compare(data, npth.read_state(odata))
+
+
+def _aniso_factors(self, deriv):
+    """Integrating factors that are not coeff |k|^(2 order): an anisotropic
+    hyperviscosity nu (kx^2 + 3 ky^2)^2 plus nu/2 |k|^2, one array per
+    velocity component (what a subclass may set; physics.py:504-513 sets
+    nu |k|^(2 vo))."""
+    nu = self.parameters['nu']
+    for _, comp in deriv['u']:
+        kx, ky = comp.k['x'], comp.k['y']
+        comp.integrating_factor = nu * (kx ** 2 + 3. * ky ** 2) ** 2 + 0.5 * nu * comp.k2()
+
+
+@pytest.mark.parametrize("shape,integ", [((32, 24), 'RK2mid'), ((32, 24), 'RK2trap'),
+                                         ((16, 12, 20), 'RK2mid'), ((16, 12, 20), 'RK2trap')],
+                         ids=str)
+def test_array_integrating_factors(shape, integ):
+    """A subclass whose integrating factors are arbitrary per-mode arrays
+    (time_step.py:289-304, 373-388 pass them to the sweeps as -IF) advances
+    through ddx_update_if: four steps against the oracle's subclass with the
+    same arrays, and the integrator's derivatives after the last one."""
+    m = mods()
+    case("array integrating factors", shape, integ)
+    Sub = type('Sub', (m.IncompressibleHydro,), {'_setup_integrating_factors': _aniso_factors})
+    OSub = type('OSub', (npth.IncompressibleHydro,), {'_setup_integrating_factors': _aniso_factors})
+    p = {'nu': 4e-3}
+    RHS = Sub(shape, m.FourierRepresentation)
+    RHS.parameters.update(p)
+    data = RHS.create_fields(0.)
+    kf = kfields('hydro', shape, 80)
+    set_fields(data, kf)
+    oph = OSub(shape)
+    oph.parameters.update(p)
+    odata = npth.load_state(oph, kf)
+    ti = (m.RK2mid if integ == 'RK2mid' else m.RK2trap)(RHS)
+    oti = (npth.RK2mid if integ == 'RK2mid' else npth.RK2trap)(oph)
+    assert not ti._fusable()
+    for _ in range(4):
+        ti.advance(data, 4e-3)
+        oti.do_advance(odata, 4e-3)
+    IF = ti.deriv1['u'][0].integrating_factor
+    assert isinstance(IF, np.ndarray)
+    compare(data, npth.read_state(odata))
+    compare(ti.deriv1, npth.read_state(oti.deriv1))
+    compare(ti.deriv2, npth.read_state(oti.deriv2))
+
+
+@pytest.mark.parametrize("f64", [True, False], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", [(24, 20), (12, 16, 10)], ids=str)
+def test_update_if_sweeps(shape, f64):
+    """ddx_update_if for each sweep kind against the oracle's sweeps
+    (forward_step_cy_{2,3}d.pyx restated) on random states and a random
+    positive factor, including |Z| < 0.5 (Taylor branch) and IF = 0 modes."""
+    m = mods()
+    from dedalus import _hip
+    from dedalus.data_objects import kgrid
+    case("update_if", shape, "f64" if f64 else "f32")
+    plan = _hip.Plan(shape, (2 * np.pi,) * len(shape), _hip.F64 if f64 else _hip.F32)
+    rng = np.random.default_rng(11)
+    ksh = kgrid.kshape(shape)
+    mask = kgrid.dealias_mask(shape, (2 * np.pi,) * len(shape))
+    IF = rng.uniform(0., 40., ksh)
+    IF[rng.random(ksh) < 0.1] = 0.
+    vals = [random_k(shape, 30 + i) for i in range(3)]
+    bufs = []
+    for v in vals:
+        b = plan.alloc_k()
+        plan.upload_k(b, v)
+        bufs.append(b)
+    ifb = plan.upload_real_k(IF)
+    out = plan.alloc_k()
+    dt = 0.03
+    nd = len(shape)
+    for kind, name in ((_hip.EULER, 'euler'), (_hip.ETD1, 'etd1'), (_hip.ETD2RK1, 'etd2rk1'),
+                       (_hip.ETD2RK2, 'etd2rk2')):
+        plan.update_if(kind, bufs[0], out, bufs[1], bufs[2], ifb, dt)
+        got = plan.download_k(out)
+        ref = np.zeros(ksh, dtype=np.complex128)
+        if name == 'euler':
+            npth.euler(vals[0], ref, vals[1], dt)
+        elif name == 'etd1':
+            npth.etd1(vals[0], ref, vals[1], -IF, dt, nd)
+        elif name == 'etd2rk1':
+            npth.etd2rk1(vals[0], ref, vals[1], vals[2], -IF, dt, nd)
+        else:
+            npth.etd2rk2(vals[0], ref, vals[1], vals[2], -IF, dt, nd)
+        ref[mask] = 0.
+        assert rel_l2(got, ref) < (1e-14 if f64 else 2e-7), (name, rel_l2(got, ref))
