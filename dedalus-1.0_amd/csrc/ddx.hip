@@ -92,12 +92,16 @@ struct AxisInfo {
   double inv = 1.0, kny = 0.0;
 };
 
-int setup_axis(int n, double L, bool half, AxisInfo* ax, std::string* why) {
+// Kept set of one axis: |k| < thr (the 2/3 rules, thr = (2/3) kny or, for
+// the spherical rule, (2/3) min(kny) of all axes), or with nyq_only every
+// mode but the Nyquist one (|k| == kny, zero_nyquist).
+int setup_axis(int n, double L, bool half, AxisInfo* ax, std::string* why, double thr_sph = 0.0,
+               bool nyq_only = false) {
   ax->n = n;
   ax->L = L;
   const double kny = (M_PI * (double)n) / L;
   const double inv = 1.0 / (double)n;
-  const double thr = 2.0 / 3.0 * kny;
+  const double thr = thr_sph > 0.0 ? thr_sph : 2.0 / 3.0 * kny;
   ax->inv = inv;
   ax->kny = kny;
   ax->kfull.resize(n);
@@ -108,7 +112,7 @@ int setup_axis(int n, double L, bool half, AxisInfo* ax, std::string* why) {
     double k = ((double)ms * inv) * 2.0 * kny;
     if (n % 2 == 0 && m == n / 2) k = -k;
     ax->kfull[m] = k;
-    keep[m] = std::fabs(k) < thr;
+    keep[m] = nyq_only ? !(std::fabs(k) == kny) : std::fabs(k) < thr;
   }
   int c = -1;
   while (c + 1 <= n / 2 && keep[c + 1]) ++c;
@@ -179,6 +183,9 @@ struct ddx_plan {
   bool zsplit = true;
   // ddx_plan_set_tuning choices, -1: the built-in rule
   int tune_wide = -1, tune_xcd = -1, tune_yepi = -1, tune_radix = -1;
+  // dealiasing rule (DDX_DEALIAS_*); the spherical rule's radius (2/3) min(kny)
+  int dealias = DDX_DEALIAS_23;
+  double sph_thr = 0.0;
   int nbatch = 1;
   long nper = 0;  // points of one transform (the 1/N of the forward transform)
   hipGraphExec_t step_exec = nullptr;
@@ -630,6 +637,36 @@ void row_defaults(const ddx_plan* p, RowArgs<T>& a) {
 // transforms
 // ---------------------------------------------------------------------------
 
+// The fused right-hand side forms the nonlinear terms in rotational form
+// (u x omega, and the induction and Lorentz terms likewise), which equals
+// the reference's convective forms after projection only when the products
+// are dealiased by a 2/3 rule; without dealiasing the aliasing errors of the
+// two forms differ.
+const char* const kNoFusedAliased =
+    "the fused right-hand side needs a 2/3 dealiasing rule (its rotational form equals the "
+    "reference's convective form only for dealiased products); with dealiasing 'None' use "
+    "the per-operation kernels";
+
+// The spherical rule's zeroing of the modes outside the sphere, in place on
+// nf compact arrays (nothing for the other rules, whose compact layout holds
+// only kept modes).
+template <typename T>
+int sphere_mask(ddx_plan* p, int nf, cpx<T>* const* f) {
+  if (p->dealias != DDX_DEALIAS_SPHERICAL || nf < 1) return 0;
+  if (nf > DDX_MAXF) return fail(DDX_ERR_ARG, "too many fields");
+  MaskArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  for (int i = 0; i < nf; ++i) a.f[i] = f[i];
+  a.g = kgeom(p);
+  a.nk = p->nk;
+  a.nf = nf;
+  a.ndim = p->ndim;
+  a.thr = p->sph_thr;
+  hipLaunchKernelGGL((sphere_mask_kernel<T>), dim3((p->nk + 255) / 256), dim3(256), 0, p->stream, a);
+  LAUNCHCHK();
+  return 0;
+}
+
 // nf fields at once: every pass is one launch over all the fields
 template <typename T>
 int fwd_impl(ddx_plan* p, int nf, const T* const* x, cpx<T>* const* kc) {
@@ -651,7 +688,8 @@ int fwd_impl(ddx_plan* p, int nf, const T* const* x, cpx<T>* const* kc) {
   if (!p->zfft) {
     for (int f = 0; f < nf; ++f) c.out[f] = kc[f];
     c.scale = T(1.0 / (double)p->nper);
-    return launch_col(p, c, p->az.n, nf, -1, false, PC_COL_FWD_Y);
+    if ((rc = launch_col(p, c, p->az.n, nf, -1, false, PC_COL_FWD_Y))) return rc;
+    return sphere_mask(p, nf, kc);
   }
   for (int f = 0; f < nf; ++f) c.out[f] = B + f * fs;
   if ((rc = launch_col(p, c, p->az.n, nf, -1, false, PC_COL_FWD_Y))) return rc;
@@ -659,13 +697,17 @@ int fwd_impl(ddx_plan* p, int nf, const T* const* x, cpx<T>* const* kc) {
   geom_fz(p, c);
   for (int f = 0; f < nf; ++f) { c.in[f] = B + f * fs; c.out[f] = kc[f]; }
   c.scale = T(1.0 / (double)p->nper);
-  return launch_col(p, c, p->Ky, nf, -1, false, PC_COL_FWD_Z);
+  if ((rc = launch_col(p, c, p->Ky, nf, -1, false, PC_COL_FWD_Z))) return rc;
+  // forward() dealiases its output (representations.py:344)
+  return sphere_mask(p, nf, kc);
 }
 
 template <typename T>
 int bwd_impl(ddx_plan* p, int nf, const cpx<T>* const* kc, T* const* x) {
   int rc = ensure_ws(p, nf);
   if (rc) return rc;
+  // backward() dealiases its input in place first (representations.py:353)
+  if ((rc = sphere_mask(p, nf, const_cast<cpx<T>* const*>(kc)))) return rc;
   const long fs = ws_field_stride(p);
   cpx<T>* A = (cpx<T>*)p->wsA;
   cpx<T>* B = (cpx<T>*)p->wsB;
@@ -870,6 +912,9 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   const long fs = ws_field_stride(p);
   cpx<T>* A = (cpx<T>*)ws.A;
   cpx<T>* B = (cpx<T>*)ws.B;
+  // the reference's first backward transforms of the state dealias it in
+  // place (representations.py:353)
+  if ((rc = sphere_mask(p, L.nstate, const_cast<cpx<T>* const*>(state)))) return rc;
 
   // backward: spectral prologue in the first column pass
   ColArgs<T> c;
@@ -941,6 +986,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   // measured as fast for 256 members)
   const bool yepi = !p->zfft && nd == 2 && (p->tune_yepi >= 0 ? p->tune_yepi == 1 : p->nbatch == 1) &&
                     p->fuse_yepi && !p->generic_only && radix_variant(p) == 0 &&
+                    p->dealias != DDX_DEALIAS_SPHERICAL &&
                     p->ay.n == 64 && p->Ky <= 2 * ((64 + 2) / 3) + 1 && L.np <= 3;
   cpx<T>* Uc;
   col_defaults(c);
@@ -962,6 +1008,14 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
     for (int o = 0; o < L.np; ++o) c.out[o] = B + o * fs;
     if ((rc = launch_col(p, c, p->az.n, L.np, -1, false, PC_COL_FWD_Y))) return rc;
     Uc = B;
+  }
+
+  // the products' forward transforms dealias them (representations.py:344);
+  // forcing, added in the epilogue, is not
+  if (p->dealias == DDX_DEALIAS_SPHERICAL) {
+    cpx<T>* uf[DDX_MAXF];
+    for (int o = 0; o < L.np; ++o) uf[o] = Uc + o * fs;
+    if ((rc = sphere_mask(p, L.np, uf))) return rc;
   }
 
   // k-space epilogue
@@ -1240,9 +1294,11 @@ namespace {
 
 // nbatch > 0: a 2D plan for nbatch independent members (z axis untransformed)
 int plan_create(int ndim, const int64_t* xshape, const double* length, int precision, int nbatch,
-                ddx_plan** out) {
+                ddx_plan** out, int dealias = DDX_DEALIAS_23) {
   if (!out || !xshape || !length) return fail(DDX_ERR_ARG, "null argument");
   *out = nullptr;
+  if (dealias != DDX_DEALIAS_23 && dealias != DDX_DEALIAS_SPHERICAL && dealias != DDX_DEALIAS_NONE)
+    return fail(DDX_ERR_ARG, "Specified dealiasing method not implemented.");
   if (g_device < 0) return fail(DDX_ERR_NODEV, "ddx_init has not been called");
   if (ndim != 2 && ndim != 3) return fail(DDX_ERR_ARG, "Must use either 2 or 3 dimensions.");
   if (precision != DDX_F64 && precision != DDX_F32) return fail(DDX_ERR_ARG, "bad precision");
@@ -1268,15 +1324,25 @@ int plan_create(int ndim, const int64_t* xshape, const double* length, int preci
   }
   p->ndim = ndim;
   p->precision = precision;
+  p->dealias = dealias;
   std::string why;
   int ny = (int)xshape[ndim - 2], nz = ndim == 3 ? (int)xshape[0] : 1;
   double lx = length[ndim - 1], ly = length[ndim - 2], lz = ndim == 3 ? length[0] : 1.0;
-  if (setup_axis(nx, lx, true, &p->ax, &why) || setup_axis(ny, ly, false, &p->ay, &why)) {
+  if (dealias == DDX_DEALIAS_SPHERICAL) {
+    // 2 / 3. * np.min(self.kny), kny = pi N / L per axis
+    double kmin = (M_PI * (double)nx) / lx;
+    kmin = std::min(kmin, (M_PI * (double)ny) / ly);
+    if (ndim == 3) kmin = std::min(kmin, (M_PI * (double)nz) / lz);
+    p->sph_thr = 2.0 / 3.0 * kmin;
+  }
+  const bool nyq = dealias == DDX_DEALIAS_NONE;
+  if (setup_axis(nx, lx, true, &p->ax, &why, p->sph_thr, nyq) ||
+      setup_axis(ny, ly, false, &p->ay, &why, p->sph_thr, nyq)) {
     delete p;
     return fail(DDX_ERR_UNSUPPORTED, why);
   }
   if (ndim == 3) {
-    if (setup_axis(nz, lz, false, &p->az, &why)) {
+    if (setup_axis(nz, lz, false, &p->az, &why, p->sph_thr, nyq)) {
       delete p;
       return fail(DDX_ERR_UNSUPPORTED, why);
     }
@@ -1362,6 +1428,11 @@ extern "C" {
 int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
                     ddx_plan** out) {
   return plan_create(ndim, xshape, length, precision, 0, out);
+}
+
+int ddx_plan_create_ex(int ndim, const int64_t* xshape, const double* length, int precision,
+                       int dealias, ddx_plan** out) {
+  return plan_create(ndim, xshape, length, precision, 0, out, dealias);
 }
 
 int ddx_plan_create_batched(int nbatch, int64_t ny, int64_t nx, const double* length, int precision,
@@ -1574,6 +1645,16 @@ int ddx_bwd_many(ddx_plan* p, int nf, const void* const* kc, void* const* x) {
   return bwd_impl<float>(p, nf, (const cpx<float>* const*)kc, (float* const*)x);
 }
 
+int ddx_dealias(ddx_plan* p, void* kc) {
+  if (check_plan(p) || !kc) return fail(DDX_ERR_ARG, "null argument");
+  if (p->precision == DDX_F64) {
+    cpx<double>* f = (cpx<double>*)kc;
+    return sphere_mask<double>(p, 1, &f);
+  }
+  cpx<float>* f = (cpx<float>*)kc;
+  return sphere_mask<float>(p, 1, &f);
+}
+
 int ddx_deriv(ddx_plan* p, const void* kc, void* out, int dim) {
   if (check_plan(p) || !kc || !out) return fail(DDX_ERR_ARG, "null argument");
   if (dim < 0 || dim >= p->ndim) return fail(DDX_ERR_ARG, "bad derivative dimension");
@@ -1649,6 +1730,7 @@ int ddx_update_if(ddx_plan* p, int kind, const void* y, void* out, const void* F
 int ddx_rhs(ddx_plan* p, const ddx_params* prm, const void* const* state, void* const* deriv,
             const void* const* forcing) {
   if (check_plan(p) || !prm || !state || !deriv) return fail(DDX_ERR_ARG, "null argument");
+  if (p->dealias == DDX_DEALIAS_NONE) return fail(DDX_ERR_UNSUPPORTED, kNoFusedAliased);
   const int nf = rhs_ws_fields(prm->physics, p->ndim);
   if (nf < 0) return fail(DDX_ERR_ARG, "unknown physics");
   int rc = ensure_ws(p, nf);
@@ -1671,6 +1753,7 @@ int step_dispatch(ddx_plan* p, int integrator, const ddx_params* prm, void* cons
                   void* const* stage, void* const* deriv1, void* const* deriv2,
                   const void* const* forcing, int nsteps, double dt, const Ws* fixed_ws) {
   if (check_plan(p) || !prm || !state || !deriv1) return fail(DDX_ERR_ARG, "null argument");
+  if (p->dealias == DDX_DEALIAS_NONE) return fail(DDX_ERR_UNSUPPORTED, kNoFusedAliased);
   if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
   if (nsteps < 0) return fail(DDX_ERR_ARG, "negative step count");
   void* const* st = stage ? stage : state;

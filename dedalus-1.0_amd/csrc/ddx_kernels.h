@@ -444,6 +444,33 @@ __global__ void update_kernel(const UpdArgs<T> a) {
   }
 }
 
+// Spherical 2/3 rule (representations.py:410-416): zero, in nf compact
+// arrays, the modes with sqrt(|k|^2) >= thr, |k|^2 summed in the reference's
+// axis order ((kx^2 + ky^2) in 2D, (ky^2 + kz^2) + kx^2 in 3D) so the
+// comparison matches its rounding.  The kept cube of the compact layout
+// holds the sphere; pad columns stay as they are (zero).
+template <typename T>
+struct MaskArgs {
+  cpx<T>* f[DDX_MAXF];
+  KGeom g;
+  long nk;
+  int nf, ndim;
+  double thr;
+};
+
+template <typename T>
+__global__ void sphere_mask_kernel(const MaskArgs<T> a) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nk) return;
+  int col;
+  double kx, ky, kz;
+  kvec<double>(a.g, idx, col, kx, ky, kz);
+  if (col >= a.g.Kx) return;
+  const double k2 = a.ndim == 2 ? kx * kx + ky * ky : (ky * ky + kz * kz) + kx * kx;
+  if (sqrt(k2) < a.thr) return;
+  for (int f = 0; f < a.nf; ++f) a.f[f][idx] = mk(T(0), T(0));
+}
+
 // Per-mode pieces of the k-space epilogue, shared by the epilogue kernel and
 // the fused forward-y epilogue of small 2D grids.  The ETD sweep of
 // components v0 <= v < v1 of one mode (update_kernel's arithmetic): phi is

@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(_HERE, "libddx.so")
 
 # enums of include/ddx.h
 F64, F32 = 0, 1
+# dealiasing rules (ddx_plan_create_ex)
+DEALIAS_23, DEALIAS_SPHERICAL, DEALIAS_NONE = 0, 1, 2
 HYDRO, BOUSSINESQ, MHD = 0, 1, 2
 RK2MID, RK2TRAP = 0, 1
 TUNE_WIDE_TILES, TUNE_XCD_ORDER, TUNE_FUSED_YEPI, TUNE_RADIX = 0, 1, 2, 3
@@ -66,6 +68,8 @@ SIGNATURES = {
     "ddx_last_error": (ctypes.c_char_p, []),
     "ddx_version": (_I, []),
     "ddx_plan_create": (_I, [_I, _I64P, _DP, _I, _PP]),
+    "ddx_plan_create_ex": (_I, [_I, _I64P, _DP, _I, _I, _PP]),
+    "ddx_dealias": (_I, [_VP, _VP]),
     "ddx_plan_create_batched": (_I, [_I, ctypes.c_int64, ctypes.c_int64, _DP, _I, _PP]),
     "ddx_plan_destroy": (_I, [_VP]),
     "ddx_plan_kshape": (_I, [_VP, _I64P, _I64P, _I64P, _I64P]),
@@ -225,13 +229,16 @@ class Plan:
     the grid at once: x arrays are (B, ny, nx), reference k arrays
     (B, nx/2+1, ny), compact k buffers hold all members."""
 
-    def __init__(self, shape, length, precision=F64, batch=None):
+    def __init__(self, shape, length, precision=F64, batch=None, dealias=DEALIAS_23):
         init()
         self.shape = tuple(int(n) for n in shape)
         self.length = tuple(float(x) for x in length)
         self.ndim = len(self.shape)
         self.precision = precision
         self.batch = int(batch) if batch else None
+        self.dealias_rule = int(dealias)
+        if self.batch and self.dealias_rule != DEALIAS_23:
+            raise ValueError("batched plans use the 2/3 rule")
         ln = (ctypes.c_double * self.ndim)(*self.length)
         p = ctypes.c_void_p()
         if self.batch:
@@ -241,7 +248,8 @@ class Plan:
                                                 precision, ctypes.byref(p)))
         else:
             sh = (ctypes.c_int64 * self.ndim)(*self.shape)
-            _check(_lib.ddx_plan_create(self.ndim, sh, ln, precision, ctypes.byref(p)))
+            _check(_lib.ddx_plan_create_ex(self.ndim, sh, ln, precision, self.dealias_rule,
+                                           ctypes.byref(p)))
         self.handle = p
         kref = (ctypes.c_int64 * 3)()
         kc = (ctypes.c_int64 * 3)()
@@ -418,6 +426,10 @@ class Plan:
                                int(F2) if F2 is not None else None,
                                float(coeff), int(order), float(dt)))
 
+    def dealias(self, kc):
+        """The plan's dealiasing rule in place on a compact k buffer."""
+        _check(_lib.ddx_dealias(self.handle, int(kc)))
+
     def update_if(self, kind, y, out, F1, F2, intfactor, dt):
         """The sweep of ``update`` with a per-mode integrating factor: a
         device buffer of nkc float64 values in the compact layout
@@ -515,11 +527,11 @@ def make_params(physics, nu=0., viscosity_order=1, kappa=0., eta=0., g=1., alpha
 _plans = {}
 
 
-def get_plan(shape, length, precision=F64):
-    """Shared plan per (shape, length, precision)."""
-    key = (tuple(int(n) for n in shape), tuple(float(x) for x in length), precision)
+def get_plan(shape, length, precision=F64, dealias=DEALIAS_23):
+    """Shared plan per (shape, length, precision, dealiasing rule)."""
+    key = (tuple(int(n) for n in shape), tuple(float(x) for x in length), precision, int(dealias))
     p = _plans.get(key)
     if p is None:
-        p = Plan(shape, length, precision)
+        p = Plan(shape, length, precision, dealias=dealias)
         _plans[key] = p
     return p

@@ -56,13 +56,24 @@ def wavenumbers(xshape, length):
     return k, kny, ksh
 
 
-def dealias_mask(xshape, length):
-    """True where the 2/3 rule removes a mode (reference k layout)."""
+def dealias_mask(xshape, length, rule='2/3'):
+    """True where a dealiasing rule removes a mode (reference k layout):
+    '2/3' (representations.py:384-397), '2/3 spherical' (:410-416: |k| >=
+    (2/3) min(kny)) or 'None' (zero_nyquist, :442-455: the Nyquist planes)."""
     k, kny, ksh = wavenumbers(xshape, length)
     kt = ktrans(len(xshape))
+    names = ('x', 'y', 'z')[:len(xshape)]
+    if rule == '2/3 spherical':
+        k2 = np.zeros(ksh)
+        for kk in k.values():
+            k2 = k2 + kk ** 2
+        return np.sqrt(k2) >= 2 / 3. * np.min(kny)
     mask = np.zeros(ksh, dtype=bool)
-    for name in ('x', 'y', 'z')[:len(xshape)]:
-        mask = mask | (np.abs(k[name]) >= 2. / 3. * kny[kt[name]])
+    for name in names:
+        if rule == 'None':
+            mask = mask | (np.abs(k[name]) == kny[kt[name]])
+        else:
+            mask = mask | (np.abs(k[name]) >= 2. / 3. * kny[kt[name]])
     return mask
 
 

@@ -9,7 +9,7 @@ assignment copies into the existing buffer; ``deriv``, ``k2``, ``k``,
 Where the data lives
 --------------------
 The canonical copy is on the device: k space as the library's compact
-array of the modes kept by the 2/3 rule, x space as float64 (z, y, x).
+array of the modes kept by the dealiasing rule, x space as float64 (z, y, x).
 Host arrays in the reference layout -- ``kdata`` (complex128, (kx, ky) /
 (ky, kz, kx)) and ``xdata`` -- are mirrors, filled on first access.  An
 array handed out by indexing or by ``.data`` / ``.kdata`` / ``.xdata``
@@ -17,8 +17,10 @@ counts as possibly modified: the next device operation on that space
 uploads it first.  Arrays obtained before a device operation are
 snapshots; index again afterwards to see the new values.
 
-Modes the 2/3 rule removes are not stored on the device, so values a
-caller writes into them survive only until the next device operation --
+Modes the dealiasing rule removes are not stored on the device (for the
+spherical rule, the compact cube holds the ball and the shell between them is
+zeroed where the reference dealiases), so values a caller writes into removed
+modes survive only until the next device operation --
 the reference clears them at its next dealias (representations.py:344,
 353); the ETD update alone would carry them along, inert, without ever
 feeding them into x space.
@@ -34,6 +36,23 @@ from dedalus.utils.parallelism import swap_indices
 from dedalus.utils.timer import timer
 
 _OPCODE = {'x': 0, 'y': 1, 'z': 2}
+
+
+# FourierRepresentation.set_dealiasing's methods (representations.py:359-382)
+# and the device rule of each; the name of the rule for kgrid.dealias_mask
+_RULES = {'2/3': (_hip.DEALIAS_23, '2/3'), '2/3 cython': (_hip.DEALIAS_23, '2/3'),
+          '2/3 spherical': (_hip.DEALIAS_SPHERICAL, '2/3 spherical'),
+          'None': (_hip.DEALIAS_NONE, 'None')}
+
+
+def dealias_rule(dealiasing):
+    """(device rule, rule name) of a dealiasing method."""
+    if dealiasing is None or (isinstance(dealiasing, int) and not isinstance(dealiasing, bool)
+                              and dealiasing == 0):
+        dealiasing = 'None'
+    if dealiasing not in _RULES:
+        raise NotImplementedError("Specified dealiasing method not implemented.")
+    return _RULES[dealiasing]
 
 
 class Representation:
@@ -91,6 +110,7 @@ class FourierRepresentation(Representation):
             self.xtrans = {'x': 1, 1: 'x', 'y': 0, 0: 'y'}
         else:
             self.xtrans = {'x': 2, 2: 'x', 'y': 1, 1: 'y', 'z': 0, 0: 'z'}
+        self._rule = dealias_rule(dealiasing)
         self.set_fft(method)
         self.set_dealiasing(dealiasing)
         self._setup_k()
@@ -115,7 +135,8 @@ class FourierRepresentation(Representation):
             raise ValueError("[hip] precision must be float64 or float32")
         self._plan = _hip.get_plan(tuple(int(n) for n in self.global_shape['xspace']),
                                    tuple(float(x) for x in self.length),
-                                   _hip.F64 if prec == 'float64' else _hip.F32)
+                                   _hip.F64 if prec == 'float64' else _hip.F32,
+                                   self._rule[0])
         self._kc = self._plan.alloc_k()
         self._xd = None
         self._dd = None
@@ -307,14 +328,28 @@ class FourierRepresentation(Representation):
         self.rev_count += 1
 
     def set_dealiasing(self, dealiasing):
+        """Assign the dealiasing method (representations.py:359-382).  The
+        device plan follows the rule; changing it moves the data through the
+        host onto a plan of the new rule."""
         mylog.debug("Setting dealiasing method to %s." % dealiasing)
-        if dealiasing in ('2/3', '2/3 cython'):
+        rule = dealias_rule(dealiasing)
+        n = self.global_shape['xspace']
+        if rule[0] == _hip.DEALIAS_23:
             self.dealias = self.dealias_23
-            self.nmodes = np.prod(2 * np.ceil(self.global_shape['xspace'] / 3. - 1) + 1)
+            self.nmodes = np.prod(2 * np.ceil(n / 3. - 1) + 1)
+        elif rule[0] == _hip.DEALIAS_SPHERICAL:
+            self.dealias = self.dealias_23_spherical
+            self.nmodes = None
         else:
-            raise NotImplementedError(
-                "dealiasing %r is not provided: the MI355X transforms apply the 2/3 rule"
-                % dealiasing)
+            self.dealias = self.zero_nyquist
+            self.nmodes = np.prod(2 * np.ceil(n / 2. - 1) + 1)
+        if rule[0] != self._rule[0]:
+            space = self._curr_space
+            arr = self._to_host(space).copy()
+            self._rule = rule
+            self._allocate_memory('hip')
+            self[space] = arr
+        self._rule = rule
 
     def dealias_23(self):
         """Orszag 2/3 rule: on the device the removed modes are never stored."""
@@ -323,8 +358,16 @@ class FourierRepresentation(Representation):
 
     dealias_23_cython = dealias_23
 
+    def dealias_23_spherical(self):
+        """Spherical 2/3 rule (representations.py:410-416), in place on the device."""
+        self.require_space('kspace')
+        self._plan.dealias(self._to_dev('kspace'))
+        self._mark('kspace')
+
     def dealias_mask(self):
-        return kgrid.dealias_mask(tuple(int(n) for n in self.global_shape['xspace']), self.length)
+        """True where this component's dealiasing rule removes a mode."""
+        return kgrid.dealias_mask(tuple(int(n) for n in self.global_shape['xspace']), self.length,
+                                  self._rule[1])
 
     def deriv(self, dim):
         """i k_dim * data, into the component's derivative buffer (representations.py:419-425)."""

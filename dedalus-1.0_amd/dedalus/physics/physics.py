@@ -388,7 +388,9 @@ class IncompressibleHydro(Physics):
         if self._first_rhs:
             self._setup_integrating_factors(deriv)
             self._first_rhs = False
-        if type(self) in FUSED_CLASSES:
+        # the fused right-hand side forms the nonlinear terms in rotational
+        # form: equal to the reference's only for 2/3-dealiased products
+        if type(self) in FUSED_CLASSES and _plan(data['u']).dealias_rule != _hip.DEALIAS_NONE:
             Physics.RHS(self, data, deriv, zero_outputs=False)
             self._fused_rhs(data, deriv)
         else:

@@ -183,6 +183,7 @@ class RKBase(TimeStepBase):
         # integrating factor of the form the fused step recomputes
         return (type(ph) in FUSED_CLASSES and not ph.forcing_functions
                 and ph._phys_id is not None
+                and self._comps(self.deriv1)[0]._plan.dealias_rule != _hip.DEALIAS_NONE
                 and all(isinstance(getattr(c, 'integrating_factor', None), (IntegratingFactor, type(None)))
                         for c in self._comps(self.deriv1)))
 

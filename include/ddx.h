@@ -89,6 +89,29 @@ int ddx_version(void);
  * (representations.py:290-316) and _setup_k (representations.py:204-233). */
 int ddx_plan_create(int ndim, const int64_t* xshape, const double* length, int precision,
                     ddx_plan** plan);
+/* Dealiasing rules of FourierRepresentation.set_dealiasing
+ * (representations.py:359-382):
+ *   DDX_DEALIAS_23         '2/3' and '2/3 cython': |k_i| < (2/3) kny_i on
+ *                          every axis (representations.py:384-408);
+ *   DDX_DEALIAS_SPHERICAL  '2/3 spherical': |k| < (2/3) min(kny)
+ *                          (representations.py:410-416); the compact layout
+ *                          is the cube |k_i| < (2/3) min(kny) that holds the
+ *                          sphere, and the modes outside the sphere are
+ *                          zeroed wherever the reference dealiases: in place
+ *                          on the input of a backward transform and of a
+ *                          right-hand side, and on the output of a forward
+ *                          transform (before forcing is added);
+ *   DDX_DEALIAS_NONE       'None' / None / 0: every mode but the Nyquist
+ *                          planes (zero_nyquist, representations.py:442-455),
+ *                          products not dealiased.  ddx_rhs and ddx_step
+ *                          refuse such plans (DDX_ERR_UNSUPPORTED): their
+ *                          rotational form of the nonlinear terms equals the
+ *                          reference's convective form only for dealiased
+ *                          products; the per-operation kernels serve them. */
+enum { DDX_DEALIAS_23 = 0, DDX_DEALIAS_SPHERICAL = 1, DDX_DEALIAS_NONE = 2 };
+/* ddx_plan_create with a dealiasing rule (ddx_plan_create uses DDX_DEALIAS_23). */
+int ddx_plan_create_ex(int ndim, const int64_t* xshape, const double* length, int precision,
+                       int dealias, ddx_plan** plan);
 /* New 2D plan for nbatch independent members on one ny x nx grid (an
  * ensemble).  No reference counterpart: the reference advances one state
  * per process; this runs many coarse or ensemble members in one set of
@@ -181,6 +204,11 @@ int ddx_bwd_many(ddx_plan* plan, int nf, const void* const* kc, void* const* x);
 
 /* ---- per-operation kernels ------------------------------------------------- */
 
+/* The plan's dealiasing rule in place on a compact k array (the
+ * reference's dealias(), representations.py:384-455): zeroes the modes
+ * outside the sphere for DDX_DEALIAS_SPHERICAL; for the other rules the
+ * compact layout already holds only the kept modes and nothing is done. */
+int ddx_dealias(ddx_plan* plan, void* kc);
 /* out = i k_dim * kc, dim 0 = x, 1 = y, 2 = z (representations.py:419-425). */
 int ddx_deriv(ddx_plan* plan, const void* kc, void* out, int dim);
 /* In-place solenoidal projection F <- F - k (k.F)/|k|^2 of ndim components

@@ -24,6 +24,13 @@ import numpy as np
 # ---------------------------------------------------------------------------
 
 
+# The dealiasing method every new Component takes, as the reference's
+# FourierRepresentation takes decfg's FFT.dealiasing at construction
+# (representations.py:117, 129): '2/3' (also '2/3 cython'), '2/3 spherical'
+# or 'None' (None and 0 too).
+DEALIASING = '2/3'
+
+
 def _swap01(seq):
     """Swap entries 0 and 1 (dedalus/utils/parallelism.py:155-173)."""
     out = list(seq)
@@ -67,6 +74,18 @@ class Component:
         self.integrating_factor = None
         self.fwd_count = 0
         self.rev_count = 0
+        self.set_dealiasing(DEALIASING)
+
+    def set_dealiasing(self, dealiasing):
+        """representations.py:359-382: the rule dealias() applies."""
+        if dealiasing in ('2/3', '2/3 cython'):
+            self.rule = '2/3'
+        elif dealiasing == '2/3 spherical':
+            self.rule = '2/3 spherical'
+        elif dealiasing in ['None', None, 0]:
+            self.rule = 'None'
+        else:
+            raise NotImplementedError("Specified dealiasing method not implemented.")
 
     # --- wavenumbers (representations.py:204-233) -------------------------
     def _setup_k(self):
@@ -99,12 +118,19 @@ class Component:
         return out
 
     def dealias_mask(self):
-        """True where the 2/3 rule zeroes a mode (representations.py:384-397)."""
+        """True where the component's rule zeroes a mode: the 2/3 rule
+        (representations.py:384-397), the spherical 2/3 rule (:410-416) or
+        the Nyquist planes of zero_nyquist (:442-455)."""
+        if self.rule == '2/3 spherical':
+            return np.sqrt(self.k2()) >= 2 / 3. * np.min(self.kny)
         names = ['x', 'y'] if self.ndim == 2 else ['x', 'y', 'z']
         mask = np.zeros(self.kshape, dtype=bool)
         for nm in names:
             i = self.ktrans[nm]
-            mask = mask | (np.abs(self.k[nm]) >= 2. / 3. * self.kny[i])
+            if self.rule == 'None':
+                mask = mask | (np.abs(self.k[nm]) == self.kny[i])
+            else:
+                mask = mask | (np.abs(self.k[nm]) >= 2. / 3. * self.kny[i])
         return mask
 
     # --- lazy space state machine (representations.py:138-166, 235-245) ---
@@ -160,7 +186,7 @@ class Component:
         self.rev_count += 1
 
     def dealias(self):
-        """Orszag 2/3 rule, in place on k data (representations.py:384-397)."""
+        """The component's rule, in place on k data (representations.py:384-455)."""
         self.require_space('kspace')
         self.kdata[self.dealias_mask()] = 0.
 

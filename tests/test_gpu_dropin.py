@@ -381,3 +381,133 @@ def test_update_if_sweeps(shape, f64):
             npth.etd2rk2(vals[0], ref, vals[1], vals[2], -IF, dt, nd)
         ref[mask] = 0.
         assert rel_l2(got, ref) < (1e-14 if f64 else 2e-7), (name, rel_l2(got, ref))
+
+
+RULE_CASES = [
+    ('hydro', (32, 32), {'nu': 2e-3}, 'RK2mid'),
+    ('hydro', (64, 48), {'nu': 2e-3}, 'RK2trap'),
+    ('hydro', (16, 16, 16), {'nu': 5e-3}, 'RK2mid'),
+    ('boussinesq', (32, 32), {'nu': 1e-3, 'kappa': 1e-3, 'boussinesq_direction': 'y'}, 'RK2mid'),
+    ('mhd', (16, 16, 16), {'nu': 1e-3, 'eta': 2e-3}, 'RK2trap'),
+    ('mhd', (32, 32, 32), {'nu': 1e-3, 'eta': 1e-3}, 'RK2mid'),
+]
+
+
+class _Dealiasing:
+    """decfg's FFT.dealiasing and the oracle's for the duration of a test."""
+
+    def __init__(self, rule):
+        self.rule = rule
+
+    def __enter__(self):
+        from dedalus.config import decfg
+        self.old = decfg.get('FFT', 'dealiasing')
+        decfg.set('FFT', 'dealiasing', self.rule)
+        self.oold = npth.DEALIASING
+        npth.DEALIASING = self.rule
+
+    def __exit__(self, *exc):
+        from dedalus.config import decfg
+        decfg.set('FFT', 'dealiasing', self.old)
+        npth.DEALIASING = self.oold
+
+
+@pytest.mark.parametrize("rule", ['2/3 spherical', 'None'])
+@pytest.mark.parametrize("kind,shape,p,integ", RULE_CASES, ids=str)
+def test_dealias_rules_advance(rule, kind, shape, p, integ):
+    """The other dealiasing methods of set_dealiasing (representations.py:
+    359-382) through the fused step: RHS on its own, then five steps, and the
+    integrator's derivatives, against the oracle under the same rule."""
+    m = mods()
+    case("dealias rule", rule, kind, shape, integ)
+    with _Dealiasing(rule):
+        cls = {'hydro': m.IncompressibleHydro, 'boussinesq': m.BoussinesqHydro,
+               'mhd': m.IncompressibleMHD}[kind]
+        kf = kfields(kind, shape, 120)
+        RHS = cls(shape, m.FourierRepresentation)
+        RHS.parameters.update(p)
+        data = RHS.create_fields(0.)
+        set_fields(data, kf)
+        assert data['u'][0]._plan.dealias_rule == {'2/3 spherical': 1, 'None': 2}[rule]
+        deriv = RHS.create_fields(0.)
+        RHS.RHS(data, deriv)
+        ph = npth.make_physics(kind, shape, **dict(p))
+        odata = npth.load_state(ph, kf)
+        oderiv = ph.create_fields()
+        ph.RHS(odata, oderiv)
+        compare(deriv, npth.read_state(oderiv))
+        RHS = cls(shape, m.FourierRepresentation)
+        RHS.parameters.update(p)
+        data = RHS.create_fields(0.)
+        set_fields(data, kf)
+        ti = (m.RK2mid if integ == 'RK2mid' else m.RK2trap)(RHS)
+        ph = npth.make_physics(kind, shape, **dict(p))
+        odata = npth.load_state(ph, kf)
+        oti = (npth.RK2mid if integ == 'RK2mid' else npth.RK2trap)(ph)
+        for _ in range(5):
+            ti.advance(data, 5e-3)
+            oti.do_advance(odata, 5e-3)
+        compare(data, npth.read_state(odata))
+        compare(ti.deriv1, npth.read_state(oti.deriv1))
+        compare(ti.deriv2, npth.read_state(oti.deriv2))
+
+
+@pytest.mark.parametrize("f64", [True, False], ids=["f64", "f32"])
+@pytest.mark.parametrize("rule", ['2/3 spherical', 'None'])
+@pytest.mark.parametrize("shape", [(64, 64), (128, 96), (32, 32, 32), (16, 24, 20)], ids=str)
+def test_dealias_rules_transforms(shape, rule, f64):
+    """Plans of each rule: forward transform of a random x field against
+    the oracle's forward() (which dealiases its output), backward of k data
+    that also fills the modes the rule removes (the spherical rule's shell
+    inside the compact cube) against backward(), which dealiases its input
+    in place first -- the device buffer is zeroed there too."""
+    from dedalus import _hip
+    from helpers import random_x
+    case("dealias transforms", shape, rule, "f64" if f64 else "f32")
+    code = {'2/3 spherical': _hip.DEALIAS_SPHERICAL, 'None': _hip.DEALIAS_NONE}[rule]
+    plan = _hip.Plan(shape, (2 * np.pi,) * len(shape), _hip.F64 if f64 else _hip.F32, dealias=code)
+    tol = 1e-13 if f64 else 2e-6
+    with _Dealiasing(rule):
+        c = npth.Component(shape, (2 * np.pi,) * len(shape))
+    x = random_x(shape, 5)
+    c['xspace'] = x
+    kref = c['kspace'].copy()
+    xd, kd = plan.alloc_x(), plan.alloc_k()
+    plan.upload_x(xd, x)
+    plan.fwd(xd, kd)
+    assert rel_l2(plan.download_k(kd), kref) < tol
+    # k input with every mode of the 2/3 cube filled (Hermitian from x space)
+    with _Dealiasing('2/3'):
+        c23 = npth.Component(shape, (2 * np.pi,) * len(shape))
+    c23['xspace'] = random_x(shape, 6)
+    kin = c23['kspace'].copy()
+    if rule == 'None':
+        c['xspace'] = random_x(shape, 6)
+        kin = c['kspace'].copy()
+    plan.upload_k(kd, kin)
+    plan.bwd(kd, xd)
+    c['kspace'] = kin
+    xref = c['xspace'].copy()
+    assert rel_l2(plan.download_x(xd), xref) < tol
+    masked = kin.copy()
+    masked[c.dealias_mask()] = 0.
+    assert rel_l2(plan.download_k(kd), masked) < tol
+
+
+def test_fused_rhs_refuses_undealiased_plans():
+    """ddx_rhs / ddx_step on a plan without dealiasing fail with
+    DDX_ERR_UNSUPPORTED before launching anything (their rotational form is
+    the reference's only for dealiased products); the drop-in then runs the
+    operator sequence instead (test_dealias_rules_advance)."""
+    from dedalus import _hip
+    case("fused rhs refuses dealias None")
+    plan = _hip.Plan((32, 32), (2 * np.pi, 2 * np.pi), _hip.F64, dealias=_hip.DEALIAS_NONE)
+    prm = _hip.make_params(_hip.HYDRO, nu=1e-3)
+    st = [plan.alloc_k() for _ in range(2)]
+    d1 = [plan.alloc_k() for _ in range(2)]
+    with pytest.raises(_hip.DDXError) as e:
+        plan.rhs(prm, st, d1)
+    assert e.value.code == -3 and "dealiasing" in str(e.value)
+    with pytest.raises(_hip.DDXError) as e:
+        plan.step(_hip.RK2TRAP, prm, st, None, d1, None, 1, 1e-3)
+    assert e.value.code == -3

@@ -632,3 +632,130 @@ def test_subclass_rhs_is_unprojected(kind, shape):
         if name != 'u':
             for a, c in zip(s[name], b[name]):
                 assert rel_l2(a, c) < 1e-14
+
+
+# --- a4: the other dealiasing rules (representations.py:359-382, 410-455) ---------
+
+def _lattice_ball_half_count(n, r2):
+    """Modes of an n^3 2pi-periodic grid's half spectrum (kx >= 0, reference
+    layout) with |k|^2 < r2, counted as integer lattice points: the full-ball
+    count by rows (for each (y, z), the x extent from an integer square
+    root), then halved with the kx = 0 plane added back."""
+    import math
+    full = plane = 0
+    h = n // 2
+    for y in range(-h + 1, h):
+        for z in range(-h + 1, h):
+            rest = r2 - y * y - z * z
+            if rest <= 0:
+                continue
+            m = math.isqrt(rest - 1)          # largest x with x^2 < rest
+            full += 2 * m + 1
+            plane += 1
+    return (full + plane) // 2
+
+
+@pytest.mark.parametrize("n", [12, 24, 30, 32])
+def test_spherical_rule_keeps_the_lattice_ball(n, monkeypatch):
+    """'2/3 spherical' keeps the modes with |k| < (2/3) min(kny) = n/3 on a
+    2pi cube: the count equals an integer lattice-point count of that ball,
+    and on a box stretched along z (larger kny there) the radius still
+    comes from the smallest kny."""
+    monkeypatch.setattr(npth, 'DEALIASING', '2/3 spherical')
+    c = npth.Component((n, n, n), (TWO_PI,) * 3)
+    r2 = (n * n + 8) // 9 if n % 3 else n * n // 9   # |k|^2 < (n/3)^2 in integers
+    assert (~c.dealias_mask()).sum() == _lattice_ball_half_count(n, r2)
+    s = npth.Component((2 * n, n, n), (TWO_PI,) * 3)  # kny_z = n: radius stays n/3
+    assert (~s.dealias_mask()).sum() == (~c.dealias_mask()).sum()
+
+
+@pytest.mark.parametrize("shape", [(8, 10), (6, 8, 10)], ids=str)
+def test_none_rule_zeroes_only_nyquist(shape, monkeypatch):
+    """dealiasing 'None' is zero_nyquist: every mode is kept but the even
+    axes' Nyquist planes; (2 ceil(N/2 - 1) + 1) per axis over the full
+    spectrum (representations.py:380)."""
+    monkeypatch.setattr(npth, 'DEALIASING', 'None')
+    c = npth.Component(shape, (TWO_PI,) * len(shape))
+    keep = ~c.dealias_mask()
+    n = np.array(shape)
+    # half spectrum: kx in [0, nx/2) and every other axis without its Nyquist index
+    assert keep.sum() == (n[-1] // 2) * np.prod(n[:-1] - 1)
+    assert np.prod(2 * np.ceil(n / 2. - 1) + 1) == np.prod(n - 1)
+
+
+def _rule_products(shape, X, Y, combine, rule):
+    """The pseudo-spectral product of band-limited fields by direct sums:
+    '2/3 spherical': out_k = sum_{p+q=k} over p, q, k inside the ball
+    |f| < n/3 (aliases of such sums fall outside the 2/3 cube, which holds
+    the ball); 'None': the cyclic convolution, k = p + q mod n, over
+    p, q, k without Nyquist index.  Reference-layout half spectra."""
+    import itertools
+    nd = len(shape)
+    n = shape[-1]
+    freq = (np.fft.fftfreq(n) * n).astype(int)
+    fx = [_full_spectrum(a, shape) for a in X]
+    fy = [_full_spectrum(a, shape) for a in Y]
+    if rule == 'None':
+        def inside(v):
+            return all(abs(f) != n // 2 for f in v)
+    else:
+        def inside(v):
+            return 9 * sum(f * f for f in v) < n * n
+    modes = [m for m in itertools.product(freq, repeat=nd) if inside(m)]
+    ncomp = len(combine([0.0] * len(X), [0.0] * len(Y)))
+    c = npth.Component(shape, (TWO_PI,) * nd)
+    out = [np.zeros(c.kshape, complex) for _ in range(ncomp)]
+    for p in modes:
+        xp = [f[tuple(v % n for v in p)] for f in fx]
+        for q in modes:
+            k = [a + b for a, b in zip(p, q)]
+            if rule == 'None':
+                k = [((v + n // 2) % n) - n // 2 for v in k]
+            if not inside(k) or k[-1] < 0:
+                continue
+            yq = [f[tuple(v % n for v in q)] for f in fy]
+            val = combine(xp, yq)
+            idx = (k[1], k[0] % n) if nd == 2 else (k[0] % n, k[1] % n, k[2])
+            for i in range(ncomp):
+                out[i][idx] += val[i]
+    return out
+
+
+@pytest.mark.parametrize("rule", ['2/3 spherical', 'None'])
+def test_cross_product_3d_under_rule(rule, monkeypatch):
+    """XcrossY (physics.py:309-354) of band-limited 8^3 vectors under the
+    spherical rule (truncated to the ball) and with no dealiasing (the
+    aliased, cyclic convolution), against direct sums."""
+    monkeypatch.setattr(npth, 'DEALIASING', rule)
+    shape = (8, 8, 8)
+    ph = _mhd_physics(shape)
+    u = [random_k(shape, 170 + i) for i in range(3)]
+    B = [random_k(shape, 180 + i) for i in range(3)]
+    data = npth.load_state(ph, {'u': u, 'B': B})
+    out = ph.create_fields()
+    ph.XcrossY(data['u'], data['B'], out['u'])
+    got = [out['u'][i]['kspace'].copy() for i in range(3)]
+
+    def cross(a, b):
+        return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    want = _rule_products(shape, u, B, cross, rule)
+    for i in range(3):
+        assert rel_l2(got[i], want[i]) < 1e-13, (rule, i)
+    # the inputs themselves were band-limited by the rule (random_k dealiases)
+    mask = npth.Component(shape, (TWO_PI,) * 3).dealias_mask()
+    assert not any(np.abs(a[mask]).max() > 0 for a in u)
+
+
+@pytest.mark.parametrize("rule", ['2/3 spherical', 'None'])
+def test_cross_product_2d_under_rule(rule, monkeypatch):
+    """The 2D vector form u_x B_y - u_y B_x of XcrossY on 12^2 under each rule."""
+    monkeypatch.setattr(npth, 'DEALIASING', rule)
+    shape = (12, 12)
+    ph = _mhd_physics(shape)
+    u = [random_k(shape, 191 + i) for i in range(2)]
+    B = [random_k(shape, 193 + i) for i in range(2)]
+    data = npth.load_state(ph, {'u': u, 'B': B})
+    e = ph.aux['mathscalar']
+    ph.XcrossY(data['u'], data['B'], e)
+    want = _rule_products(shape, u, B, lambda a, b: [a[0] * b[1] - a[1] * b[0]], rule)
+    assert rel_l2(e['kspace'], want[0]) < 1e-13
