@@ -902,34 +902,49 @@ __device__ inline int wrapm(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
+// A layer's weights and bias in registers: the weights as MFMA A operands
+// (lane (m, kq) holds output channel j*16 + m, input channels kq*8.. (or
+// kq*4..) of tap t), the bias of the four output channels a lane's D values
+// hold.  Loaded one layer ahead, so the loads run under the previous layer's
+// arithmetic instead of stalling each layer's start.
+template <int CI, int CO>
+struct NetWeights {
+  static constexpr int NCO = CO / 16;
+  typedef typename std::conditional<CI == 32, bf16x8, bf16x4>::type wvec;
+  wvec w[9][NCO];
+  float b[NCO][4];
+  __device__ __forceinline__ void load(const uint16_t* w_l, const float* b_l) {
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int j = 0; j < NCO; ++j) {
+        const uint16_t* q = w_l + ((j * 16 + m) * 9 + t) * CI;
+        if constexpr (CI == 32) w[t][j] = gld8(q + 8 * kq);
+        else w[t][j] = gld4(q + 4 * kq);
+      }
+#pragma unroll
+    for (int j = 0; j < NCO; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[j][i] = b_l[j * 16 + 4 * kq + i];
+  }
+};
+
 // One layer over an output region of w x h pixels (row-major, flattened into
 // 16-pixel fragments; the last fragment's spare lanes repeat the last pixel
 // and store nothing).  in: (w + 2) x (h + 2) pixels with stride psi;
 // LAST layers write the tile's pixels to global memory.
 template <int CI, int CO, bool LAST, int NT>
-__device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* w_l, const float* b_l,
+__device__ __forceinline__ void net_layer(const NetArgs& a, const NetWeights<CI, CO>& wr,
                                           const uint16_t* in, int psi, uint16_t* outb, int pso,
                                           int w, int h, int b, int y0, int x0) {
   constexpr int NCO = CO / 16;
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  // weights as A operands, all nine taps, straight from global memory (L2)
-  typedef typename std::conditional<CI == 32, bf16x8, bf16x4>::type wvec;
-  wvec wv[9][NCO];
-#pragma unroll
-  for (int t = 0; t < 9; ++t)
-#pragma unroll
-    for (int j = 0; j < NCO; ++j) {
-      const uint16_t* q = w_l + ((j * 16 + m) * 9 + t) * CI;
-      if constexpr (CI == 32) wv[t][j] = gld8(q + 8 * kq);
-      else wv[t][j] = gld4(q + 4 * kq);
-    }
-  float bv[NCO][4];
-#pragma unroll
-  for (int j = 0; j < NCO; ++j)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) bv[j][i] = b_l[j * 16 + 4 * kq + i];
+  const auto& wv = wr.w;
+  const auto& bv = wr.b;
   const int npx = w * h, nfrag = (npx + 15) / 16;
   const int wi = w + 2;
   const int npairs = (nfrag + 1) / 2;
@@ -1032,22 +1047,34 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* w_l,
   if (pendB) finish(accB, ppB, vB);
 }
 
+template <int L, int NL, int HID>
+struct NetLayerShape {
+  static constexpr int CI = L == 0 ? 16 : HID;
+  static constexpr int CO = L == NL - 1 ? 16 : HID;
+  typedef NetWeights<CI, CO> W;
+};
+
+// Layer L with its weights already in registers: the next layer's weights
+// are loaded first, then layer L runs, then the next layer after a barrier.
 template <int L, int NL, int HID, int TX, int TY, int NT>
 __device__ __forceinline__ void net_layers(const NetArgs& a, uint16_t* buf0, uint16_t* buf1, int b,
-                                           int y0, int x0) {
-  if constexpr (L < NL) {
-    using P = NetPlan<NL, HID, TX, TY>;
-    constexpr int CI = L == 0 ? 16 : HID;
-    constexpr int CO = L == NL - 1 ? 16 : HID;
-    constexpr int g = NL - 1 - L;  // output halo
-    const uint16_t* in = (L % 2 == 0) ? buf0 : buf1;
-    uint16_t* outb = (L % 2 == 0) ? buf1 : buf0;
-    net_layer<CI, CO, L == NL - 1, NT>(a, a.w[L], a.bias[L], in, L == 0 ? P::PSI : P::PSH, outb,
-                                       P::PSH, TX + 2 * g, TY + 2 * g, b, y0, x0);
-    if constexpr (L < NL - 1) {
-      lds_barrier();
-      net_layers<L + 1, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0);
-    }
+                                           int y0, int x0,
+                                           const typename NetLayerShape<L, NL, HID>::W& cur) {
+  using P = NetPlan<NL, HID, TX, TY>;
+  using S = NetLayerShape<L, NL, HID>;
+  constexpr int g = NL - 1 - L;  // output halo
+  const uint16_t* in = (L % 2 == 0) ? buf0 : buf1;
+  uint16_t* outb = (L % 2 == 0) ? buf1 : buf0;
+  if constexpr (L < NL - 1) {
+    typename NetLayerShape<L + 1, NL, HID>::W nxt;
+    nxt.load(a.w[L + 1], a.bias[L + 1]);
+    net_layer<S::CI, S::CO, false, NT>(a, cur, in, L == 0 ? P::PSI : P::PSH, outb, P::PSH,
+                                       TX + 2 * g, TY + 2 * g, b, y0, x0);
+    lds_barrier();
+    net_layers<L + 1, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0, nxt);
+  } else {
+    net_layer<S::CI, S::CO, true, NT>(a, cur, in, P::PSH, outb, P::PSH, TX + 2 * g, TY + 2 * g, b,
+                                      y0, x0);
   }
 }
 
@@ -1064,6 +1091,9 @@ __global__ void __launch_bounds__(NT) conv_net_kernel(NetArgs a) {
   const int ty = rest % tiles_y;
   const int b = rest / tiles_y;
   const int x0 = tx * TX, y0 = ty * TY;
+  // the first layer's weights load while the input is staged
+  typename NetLayerShape<0, NL, HID>::W w0;
+  w0.load(a.w[0], a.bias[0]);
   // stage the input region with a halo of NL pixels (periodic wrap)
   constexpr int RW = TX + 2 * NL, RH = TY + 2 * NL;
   for (int i = threadIdx.x; i < RW * RH; i += NT) {
@@ -1097,7 +1127,7 @@ __global__ void __launch_bounds__(NT) conv_net_kernel(NetArgs a) {
     d[3] = make_uint2(q1.z, q1.w);
   }
   lds_barrier();
-  net_layers<0, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0);
+  net_layers<0, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0, w0);
 }
 
 template <int NL, int HID, int TX, int TY, int NT>
