@@ -22,8 +22,15 @@ Fields of the JSON line besides the standard ones:
                 pair per launch); traffic = HBM bytes per launch from the
                 rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/ when
                 one exists for this kernel and workload, else null
-  step_roofline the whole step against the byte model of B_alg per
-                point-step (compulsory traffic of the minimal dataflow)
+  step_algorithmic  the whole step's algorithmic bytes (every kernel's
+                input and output arrays once, summed over the instrumented
+                pass, per step) x K / wall of the timed region, against
+                8 TB/s: the step's roofline fraction on the bytes this
+                dataflow must move
+  model_speed   B_model per point-step (the byte model of the reference's
+                minimal dataflow: 27 transforms on full layouts) x points x
+                K / wall, against 8 TB/s: a speed ratio against that model,
+                not a fraction of measured bytes (this dataflow moves fewer)
   hbm_measured  the metric's %HBM: HBM bytes per step measured by
                 rocprofv3 FETCH_SIZE + WRITE_SIZE (the step summary in
                 profiles/ for this workload and precision) x K / wall of
@@ -57,8 +64,9 @@ sys.path.insert(0, os.path.join(ROOT, "dedalus-1.0_amd"))
 METRIC = "Mpt-steps/s at 256³ FP64 + %HBM peak; closure speedup at equal spectrum err"
 HBM_PEAK_GBS = 8000.0
 
-# compulsory bytes per point-step of the minimal dataflow (FP64), per workload
-B_ALG_PER_POINT_STEP = {
+# bytes per point-step of the byte model of the reference's minimal dataflow
+# (27 transforms on full layouts for 3D MHD; FP64), per workload
+B_MODEL_PER_POINT_STEP = {
     "mhd3d_256": 1206.0,
     "hydro3d_128": 654.4,
     "boussinesq2d_512": 521.3,
@@ -568,14 +576,18 @@ def main():
     kernels = {k: {"launches": v[0], "ms": round(v[1], 3),
                    "GBs": round(v[2] / (v[1] / 1e3) / 1e9, 1) if v[1] > 0 else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-    balg = B_ALG_PER_POINT_STEP.get(w["name"])
+    balg = B_MODEL_PER_POINT_STEP.get(w["name"])
     if balg and args.precision == "f32":
         balg = balg / 2.0
-    step_roof = None
+    model = None
     if balg:
         gbs = balg * npts * args.steps / wall / 1e9
-        step_roof = {"bytes_per_point_step": balg, "achieved": round(gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        model = {"bytes_per_point_step": balg, "achieved": round(gbs, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "ratio": round(gbs / HBM_PEAK_GBS, 4)}
+    alg_step = sum(v[2] for v in prof.values()) / args.steps
+    gbs = alg_step * args.steps / wall / 1e9
+    step_alg = {"bytes_per_step": alg_step, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
     hbm_meas = None
     step_bytes = step_traffic_from_profiles(w["name"], args.precision)
@@ -604,7 +616,8 @@ def main():
                    "grid": list(w["shape"]), "integrator": w["integrator"], "dt": w["dt"],
                    "params": w["params"]},
         "roofline": roof,
-        "step_roofline": step_roof,
+        "step_algorithmic": step_alg,
+        "model_speed": model,
         "hbm_measured": hbm_meas,
         "kernels": kernels,
         "source_hash": source_hash(),

@@ -992,7 +992,9 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const NetWeights<CI,
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       if (!LAST) {
-        if (!valid[u]) continue;
+        // no branch on valid[u]: a spare lane computed the clamped last pixel
+        // with the same operands as the lane that owns it, so its store
+        // writes the same bits to the same LDS address
 #pragma unroll
         for (int j = 0; j < NCO; ++j) {
           uint32_t h2[2];
