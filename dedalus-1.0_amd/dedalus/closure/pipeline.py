@@ -292,6 +292,7 @@ class CoarseRun:
         self.bufs = [[C.alloc_k() for _ in range(2)] for _ in range(2)]
         self.net = net
         self.graph = None
+        self.graphs = []
         m = pipe.coarse_shape
         nb = self.batch or 1
         self.npix = nb * m[0] * m[1]
@@ -341,27 +342,40 @@ class CoarseRun:
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,
                1, self.dt, forcing=forcing)
 
-    def capture(self):
-        """Record one step as a graph (after one eager warm-up step)."""
+    def capture(self, steps_per_graph=10):
+        """Record the steps as graphs (after one eager warm-up step): one of
+        ``steps_per_graph`` consecutive steps, replayed for the bulk of a run
+        so that the per-replay launch cost is paid once per that many steps,
+        and one of a single step for the remainder."""
         C = self.C
         C.set_graph(False)   # the step's own graph would nest inside this capture
+        k = max(1, int(steps_per_graph))
+        graphs = []
         with torch.cuda.stream(self.pipe.stream):
             self.one_step()
             self.pipe.stream.synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=self.pipe.stream):
-                self.one_step()
+            for n in sorted({1, k}):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=self.pipe.stream):
+                    for _ in range(n):
+                        self.one_step()
+                graphs.append((n, g))
         self.pipe.stream.synchronize()
-        self.graph = g
-        return g
+        self.graphs = sorted(graphs, key=lambda t: -t[0])
+        self.graph = self.graphs[-1][1]
+        return self.graph
 
     def advance(self, nsteps):
+        n = int(nsteps)
         with torch.cuda.stream(self.pipe.stream):
-            for _ in range(int(nsteps)):
-                if self.graph is not None:
-                    self.graph.replay()
-                else:
+            if self.graph is None:
+                for _ in range(n):
                     self.one_step()
+                return
+            for k, g in self.graphs:
+                while n >= k:
+                    g.replay()
+                    n -= k
 
 
 def spectrum_of(plan, bufs, shape, length):

@@ -79,9 +79,10 @@ def _pipe_small():
 
 
 def test_coarse_graph_replays_match_eager():
-    """The captured coarse closure step (features, network, forcing transform,
-    RK2mid step) replays bit-identically to eager steps, with host reads and
-    other device work between replays."""
+    """The captured coarse closure steps (features, network, forcing
+    transform, RK2mid step), one step per graph and three steps per graph,
+    replay bit-identically to eager steps, with host reads and other device
+    work between replays."""
     import torch
     from dedalus.closure import ClosureNet
     from dedalus.closure.sgs import initial_velocity
@@ -97,12 +98,12 @@ def test_coarse_graph_replays_match_eager():
         C.upload_k(b, a)
         ub.append(b)
     results = []
-    for graph in (False, True):
+    for graph in (0, 1, 3):
         for with_net in (False, True):
             r = pipe.coarse_run(net if with_net else None, scales if with_net else None)
             r.set_state(ub)
             if graph:
-                r.capture()
+                r.capture(steps_per_graph=graph)
                 r.set_state(ub)
             else:
                 C.set_graph(False)
@@ -114,7 +115,7 @@ def test_coarse_graph_replays_match_eager():
                 snaps.append([C.download_k(b) for b in r.state])
                 _ = torch.ones(1000, device="cuda").sum().item()   # other device work
             results.append(snaps)
-    for a, b in ((0, 2), (1, 3)):
+    for a, b in ((0, 2), (1, 3), (0, 4), (1, 5)):
         for sa, sb in zip(results[a], results[b]):
             for x, y in zip(sa, sb):
                 assert np.isfinite(x).all()
