@@ -227,14 +227,54 @@ class Pipeline:
         return CoarseRun(self, net, scales, batch, fused)
 
 
-def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=None):
+def square_symmetry(A, g, pseudo=()):
+    """Element g (0..7) of the square's symmetry group applied to a batch of
+    x-space images (B, H, W, C) on an H = W periodic grid whose channels 0, 1
+    are the x and y components of a vector, and channels in ``pseudo`` change
+    sign under reflections (the vorticity).  Bit 0: transpose (x <-> y), bit
+    1: reflect x (column j -> W - 1 - j), bit 2: reflect y.  Reflections about
+    the grid's centre map the 8 x 8 blocks of the box filter onto each other,
+    so the sub-grid target is exactly equivariant under them."""
+    out = A
+    sign = torch.ones(A.shape[-1], device=A.device, dtype=A.dtype)
+    if g & 1:
+        out = out.transpose(1, 2)
+        perm = list(range(A.shape[-1]))
+        perm[0], perm[1] = 1, 0
+        out = out[..., perm]
+        for c in pseudo:
+            sign[c] = -sign[c]
+    if g & 2:
+        out = out.flip(2)
+        sign[0] = -sign[0]
+        for c in pseudo:
+            sign[c] = -sign[c]
+    if g & 4:
+        out = out.flip(1)
+        sign[1] = -sign[1]
+        for c in pseudo:
+            sign[c] = -sign[c]
+    return (out * sign).contiguous()
+
+
+def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=None,
+          augment=True):
     """Adam on the normalised MSE; returns (scales, history).  The last
-    val_frac of the pairs (the latest samples) is held out."""
+    val_frac of the pairs (the latest samples) is held out.  X channels are
+    (u, v, omega), Y channels the sub-grid tendency (tau_u, tau_v).  With
+    ``augment`` each batch is mapped by a random symmetry of the square
+    (``square_symmetry``): the sub-grid tendency of 2D turbulence on a
+    square periodic box, box-filtered on aligned blocks, is equivariant
+    under them, so the 2000 pairs act as up to 8 x 2000.  The velocity
+    components then share one normalisation scale."""
     n = X.shape[0]
     nval = max(1, int(round(n * val_frac)))
     ntr = n - nval
     xs = X[:ntr].std(dim=(0, 1, 2)).clamp_min(1e-12)
     ys = Y[:ntr].std(dim=(0, 1, 2)).clamp_min(1e-12)
+    if augment:
+        xs[:2] = xs[:2].square().mean().sqrt()
+        ys[:2] = ys[:2].square().mean().sqrt()
     Xn, Yn = X / xs, Y / ys
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=iters, eta_min=lr * 0.02)
@@ -243,8 +283,12 @@ def train(net, X, Y, iters=3000, batch=32, lr=2e-3, seed=0, val_frac=0.1, log=No
     t0 = time.perf_counter()
     for it in range(iters):
         idx = torch.randint(0, ntr, (batch,), generator=gen).to(X.device)
+        xb, yb = Xn[idx], Yn[idx]
+        if augment:
+            g = int(torch.randint(0, 8, (1,), generator=gen))
+            xb, yb = square_symmetry(xb, g, pseudo=(2,)), square_symmetry(yb, g)
         opt.zero_grad(set_to_none=True)
-        loss = ((net(Xn[idx]) - Yn[idx]) ** 2).mean()
+        loss = ((net(xb) - yb) ** 2).mean()
         loss.backward()
         opt.step()
         sched.step()
