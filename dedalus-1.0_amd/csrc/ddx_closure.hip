@@ -1435,14 +1435,19 @@ int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
   const bool small_ok = H % 8 == 0 && W % 16 == 0;
   if (nl == 5 && hid == 32) {
     if (big >= 256 || (big > 0 && !small_ok)) return launch_net<5, 32, 32, 16, 512>(a, s);
+    // a few images (a single coarse run): 8 x 8 tiles, each layer's region
+    // at most one fragment pair per wavefront, so the five layers are five
+    // short dependent phases rather than rounds of pairs
+    const long small = small_ok ? (long)B * (H / 8) * (W / 16) : 0;
+    if (small < 128 && H % 8 == 0 && W % 8 == 0) return launch_net<5, 32, 8, 8, 512>(a, s);
     if (small_ok) return launch_net<5, 32, 16, 8, 256>(a, s);
   } else if (nl == 3 && hid == 16) {
     if (big >= 256 || (big > 0 && !small_ok)) return launch_net<3, 16, 32, 16, 256>(a, s);
     if (small_ok) return launch_net<3, 16, 16, 8, 256>(a, s);
   }
   return ddx::set_error(DDX_ERR_UNSUPPORTED,
-                        "fused network: depth 5 width 32 or depth 3 width 16 on images of "
-                        "multiples of 8 x 16 pixels");
+                        "fused network: depth 5 width 32 on images of multiples of 8 x 8 "
+                        "pixels, or depth 3 width 16 on multiples of 8 x 16");
 }
 
 int ddx_mfma_probe(void* stream, int k, const void* A, const void* Bm, float* D) {

@@ -114,10 +114,13 @@ int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void
  * NULL) chans[0] float64 fields src[c][p] scaled by in_scale[c] and rounded
  * to bf16 as ddx_conv_pack does.  Output: out [npix][16] fp32, or (out NULL)
  * chans[nl] float64 fields dst[c][p] = out_scale[c] * y_c[p] (+ add[c][p])
- * as ddx_conv_unpack does.  Supported: depth 5 with 32 hidden channels and
- * depth 3 with 16, inputs and outputs of at most 16 channels, H a multiple
- * of 8 and W of 16; otherwise DDX_ERR_UNSUPPORTED and the layer-by-layer
- * calls apply.  in_scale, out_scale, add may be NULL (1, 1, nothing). */
+ * as ddx_conv_unpack does.  Supported: depth 5 with 32 hidden channels (H
+ * and W multiples of 8) and depth 3 with 16 (H a multiple of 8, W of 16),
+ * inputs and outputs of at most 16 channels; otherwise DDX_ERR_UNSUPPORTED
+ * and the layer-by-layer calls apply.  The tile size follows the image
+ * count: 32 x 16 output tiles when they alone fill the chip, 8 x 8 for a few
+ * images (one coarse run), 16 x 8 in between.  in_scale, out_scale, add may
+ * be NULL (1, 1, nothing). */
 int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
                  const void* const* w, const float* const* bias, const void* x, float* out,
                  const void* const* src, const float* in_scale, void* const* dst,
