@@ -735,6 +735,45 @@ int bwd_impl(ddx_plan* p, int nf, const cpx<T>* const* kc, T* const* x) {
   return launch_row(p, r);
 }
 
+// Backward transforms of linear combinations of compact k arrays: the
+// combination (recipes, as the RHS's derived fields) is formed in the first
+// column pass, so derivative fields need no k-space pass of their own.
+template <typename T>
+int bwd_combine_impl(ddx_plan* p, int nsrc, const cpx<T>* const* kc, int nout, const Recipe* rec,
+                     T* const* x) {
+  int rc = ensure_ws(p, nout);
+  if (rc) return rc;
+  // backward() dealiases what it transforms in place (representations.py:353)
+  if ((rc = sphere_mask(p, nsrc, const_cast<cpx<T>* const*>(kc)))) return rc;
+  const long fs = ws_field_stride(p);
+  cpx<T>* A = (cpx<T>*)p->wsA;
+  cpx<T>* B = (cpx<T>*)p->wsB;
+  ColArgs<T> c;
+  col_defaults(c);
+  for (int s = 0; s < nsrc; ++s) c.src[s] = kc[s];
+  for (int q = 0; q < nout; ++q) c.rec[q] = rec[q];
+  if (p->zfft) {
+    geom_bz(p, c);
+    for (int q = 0; q < nout; ++q) c.out[q] = A + q * fs;
+    if ((rc = launch_col(p, c, p->Ky, nout, 1, true, PC_COL_BWD_Z_SPECTRAL, nsrc))) return rc;
+    col_defaults(c);
+    geom_by(p, c);
+    for (int q = 0; q < nout; ++q) { c.in[q] = A + q * fs; c.out[q] = B + q * fs; }
+    if ((rc = launch_col(p, c, p->az.n, nout, 1, false, PC_COL_BWD_Y))) return rc;
+  } else {
+    geom_by(p, c);
+    for (int q = 0; q < nout; ++q) c.out[q] = B + q * fs;
+    if ((rc = launch_col(p, c, p->az.n, nout, 1, true, PC_COL_BWD_Y_SPECTRAL, nsrc))) return rc;
+  }
+  RowArgs<T> r;
+  row_defaults(p, r);
+  r.mode = ROW_C2R;
+  r.nin = nout;
+  r.nout = 0;
+  for (int q = 0; q < nout; ++q) { r.in[q] = B + q * fs; r.xout[q] = x[q]; }
+  return launch_row(p, r);
+}
+
 // ---------------------------------------------------------------------------
 // fused RHS
 // ---------------------------------------------------------------------------
@@ -1643,6 +1682,34 @@ int ddx_bwd_many(ddx_plan* p, int nf, const void* const* kc, void* const* x) {
   if (p->precision == DDX_F64)
     return bwd_impl<double>(p, nf, (const cpx<double>* const*)kc, (double* const*)x);
   return bwd_impl<float>(p, nf, (const cpx<float>* const*)kc, (float* const*)x);
+}
+
+int ddx_bwd_combine(ddx_plan* p, int nsrc, const void* const* kc, int nout, const int* nterm,
+                    const int* src, const int* dim, const double* coef, void* const* x) {
+  if (check_plan(p) || !kc || !x || !nterm || !src || !dim || !coef)
+    return fail(DDX_ERR_ARG, "null argument");
+  if (nsrc < 1 || nsrc > DDX_MAXF || nout < 1 || nout > DDX_MAXF)
+    return fail(DDX_ERR_ARG, "1..16 sources and outputs");
+  for (int s = 0; s < nsrc; ++s)
+    if (!kc[s]) return fail(DDX_ERR_ARG, "null source");
+  Recipe rec[DDX_MAXF];
+  memset(rec, 0, sizeof(rec));
+  for (int q = 0; q < nout; ++q) {
+    if (!x[q]) return fail(DDX_ERR_ARG, "null output");
+    if (nterm[q] < 1 || nterm[q] > 2) return fail(DDX_ERR_ARG, "1 or 2 terms per output");
+    rec[q].nterm = nterm[q];
+    for (int t = 0; t < nterm[q]; ++t) {
+      const int i = 2 * q + t;
+      if (src[i] < 0 || src[i] >= nsrc) return fail(DDX_ERR_ARG, "term source out of range");
+      if (dim[i] < -1 || dim[i] >= p->ndim) return fail(DDX_ERR_ARG, "term dimension out of range");
+      rec[q].t[t].src = src[i];
+      rec[q].t[t].dim = dim[i];
+      rec[q].t[t].coef = coef[i];
+    }
+  }
+  if (p->precision == DDX_F64)
+    return bwd_combine_impl<double>(p, nsrc, (const cpx<double>* const*)kc, nout, rec, (double* const*)x);
+  return bwd_combine_impl<float>(p, nsrc, (const cpx<float>* const*)kc, nout, rec, (float* const*)x);
 }
 
 int ddx_dealias(ddx_plan* p, void* kc) {

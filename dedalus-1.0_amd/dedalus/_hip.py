@@ -62,6 +62,7 @@ _SZ = ctypes.c_size_t
 _PP = ctypes.POINTER(ctypes.c_void_p)
 _I64P = ctypes.POINTER(ctypes.c_int64)
 _DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
 
 SIGNATURES = {
     "ddx_init": (_I, [_I]),
@@ -70,6 +71,7 @@ SIGNATURES = {
     "ddx_plan_create": (_I, [_I, _I64P, _DP, _I, _PP]),
     "ddx_plan_create_ex": (_I, [_I, _I64P, _DP, _I, _I, _PP]),
     "ddx_dealias": (_I, [_VP, _VP]),
+    "ddx_bwd_combine": (_I, [_VP, _I, _PP, _I, _IP, _IP, _IP, _DP, _PP]),
     "ddx_plan_create_batched": (_I, [_I, ctypes.c_int64, ctypes.c_int64, _DP, _I, _PP]),
     "ddx_plan_destroy": (_I, [_VP]),
     "ddx_plan_kshape": (_I, [_VP, _I64P, _I64P, _I64P, _I64P]),
@@ -425,6 +427,22 @@ class Plan:
         _check(_lib.ddx_update(self.handle, kind, int(y), int(out), int(F1),
                                int(F2) if F2 is not None else None,
                                float(coeff), int(order), float(dt)))
+
+    def bwd_combine(self, kcs, recipes, xs):
+        """x-space outputs of linear combinations of compact k buffers:
+        recipes[q] = [(src index, dim (-1: none, 0/1/2: i k_x/y/z), coef), ...]
+        with one or two terms; xs[q] receives the backward transform of the
+        q-th combination."""
+        n = len(recipes)
+        nterm = (ctypes.c_int * n)(*[len(r) for r in recipes])
+        src = (ctypes.c_int * (2 * n))()
+        dim = (ctypes.c_int * (2 * n))()
+        coef = (ctypes.c_double * (2 * n))()
+        for q, r in enumerate(recipes):
+            for t, (s_, d_, c_) in enumerate(r):
+                src[2 * q + t], dim[2 * q + t], coef[2 * q + t] = int(s_), int(d_), float(c_)
+        _check(_lib.ddx_bwd_combine(self.handle, len(kcs), ptr_array([int(k) for k in kcs]), n,
+                                    nterm, src, dim, coef, ptr_array([int(x) for x in xs])))
 
     def dealias(self, kc):
         """The plan's dealiasing rule in place on a compact k buffer."""

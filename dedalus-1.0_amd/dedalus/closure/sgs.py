@@ -107,11 +107,15 @@ def vorticity_k(plan, u, out):
     plan.kcombine(out, [(u[1], 1.0, 0), (u[0], -1.0, 1)])
 
 
-def features_x(plan, u, xbufs, work_k):
-    """x-space u, v, omega of a 2D velocity into three x buffers of the plan
-    (the three backward transforms as one call)."""
-    vorticity_k(plan, u, work_k)
-    plan.bwd_many([u[0], u[1], work_k], xbufs[:3])
+# the features' recipes: u, v and omega = d_x v - d_y u
+FEATURE_RECIPES = [[(0, -1, 1.0)], [(1, -1, 1.0)], [(1, 0, 1.0), (0, 1, -1.0)]]
+
+
+def features_x(plan, u, xbufs, work_k=None):
+    """x-space u, v, omega of a 2D velocity into three x buffers of the plan:
+    one backward transform call that forms omega in its first column pass
+    (work_k is unused; omega needs no k-space array of its own)."""
+    plan.bwd_combine([u[0], u[1]], FEATURE_RECIPES, xbufs[:3])
 
 
 def shell_spectrum(kref_comps, shape, length):

@@ -201,6 +201,16 @@ int ddx_bwd(ddx_plan* plan, const void* kc, void* x);
  * launches, not bytes, set the time. */
 int ddx_fwd_many(ddx_plan* plan, int nf, const void* const* x, void* const* kc);
 int ddx_bwd_many(ddx_plan* plan, int nf, const void* const* kc, void* const* x);
+/* Backward transforms of linear combinations of nsrc compact k arrays, the
+ * derived fields the reference forms with deriv() and then transforms
+ * (e.g. the vorticity of curlX, physics.py:374-405, representations.py:
+ * 419-425 then 347-357), in one pass: output q (< nout) is
+ *   bwd( sum_{t < nterm[q]} coef[2q+t] * M(dim[2q+t]) * kc[src[2q+t]] )
+ * with M = identity (dim -1) or i k_x / i k_y / i k_z (dim 0 / 1 / 2) and
+ * nterm[q] 1 or 2.  The combination is formed in the first column pass, as
+ * the fused right-hand side forms its derived fields. */
+int ddx_bwd_combine(ddx_plan* plan, int nsrc, const void* const* kc, int nout, const int* nterm,
+                    const int* src, const int* dim, const double* coef, void* const* x);
 
 /* ---- per-operation kernels ------------------------------------------------- */
 

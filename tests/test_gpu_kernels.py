@@ -668,3 +668,45 @@ def test_radix_variant_fp32():
     plan.step(hp.RK2MID, params_struct(hp, 'hydro', 2, {'nu': 1e-3}), st, bufs[0], bufs[1], None, 3, 2e-3)
     for c in range(2):
         assert rel_l2(plan.download_k(st[c]), ref['u'][c]) < 2e-6
+
+
+@pytest.mark.parametrize("f64", [True, False], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", [(64, 64), (48, 64), (30, 20), (16, 16, 16), (12, 10, 16),
+                                   (4, 6, 128)], ids=str)
+def test_bwd_combine(shape, f64):
+    """ddx_bwd_combine against the oracle's deriv() then backward(): the
+    field itself, one derivative with a coefficient, and curl-type
+    differences of two sources (the vorticity of curlX, physics.py:374-405)."""
+    hp = hip()
+    nd = len(shape)
+    L = LENGTHS[nd]
+    case("bwd_combine", shape, "f64" if f64 else "f32")
+    plan = hp.Plan(shape, L, hp.F64 if f64 else hp.F32)
+    srcs = [random_k(shape, 400 + i, L) for i in range(nd)]
+    ks = []
+    for a in srcs:
+        b = plan.alloc_k()
+        plan.upload_k(b, a)
+        ks.append(b)
+    names = ['x', 'y', 'z'][:nd]
+    recipes = [[(0, -1, 1.0)], [(1, 0, -2.5)],
+               [(1, 0, 1.0), (0, 1, -1.0)]]
+    if nd == 3:
+        recipes += [[(2, 1, 1.0), (1, 2, -1.0)], [(0, 2, 0.5), (2, -1, 3.0)]]
+    xs = [plan.alloc_x() for _ in recipes]
+    plan.bwd_combine(ks, recipes, xs)
+    comps = []
+    for a in srcs:
+        c = npth.Component(shape, L)
+        c['kspace'] = a
+        comps.append(c)
+    for q, r in enumerate(recipes):
+        acc = np.zeros_like(srcs[0])
+        for s_, d_, c_ in r:
+            term = comps[s_]['kspace'].copy() if d_ < 0 else comps[s_].deriv(names[d_]).copy()
+            acc = acc + c_ * term
+        out = npth.Component(shape, L)
+        out['kspace'] = acc
+        want = out['xspace']
+        got = plan.download_x(xs[q])
+        assert rel_l2(got, want) < (1e-13 if f64 else 2e-6), (q, rel_l2(got, want))
