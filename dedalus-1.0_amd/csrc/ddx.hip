@@ -511,6 +511,15 @@ void geom_fz(const ddx_plan* p, ColArgs<T>& a) {
 
 template <typename T, int N, int PH, int ND>
 void row_fast_go(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  bool xf = false;
+  for (int f = 0; f < a.nout; ++f) xf = xf || a.xadd[f] != nullptr;
+  if (xf) {
+    // x-space forcing: the base radix plan with the forcing loads
+    constexpr int RPB = 64 / RowPlan<N>::LANES;
+    hipLaunchKernelGGL((row_phys_fast<T, N, PH, ND, 0, true>), dim3((nrows + RPB - 1) / RPB),
+                       dim3(64), 0, p->stream, a, nrows);
+    return;
+  }
   if (radix_variant(p)) {
     constexpr int RPB = 64 / RowPlanV<N, 1>::LANES;
     hipLaunchKernelGGL((row_phys_fast<T, N, PH, ND, 1>), dim3((nrows + RPB - 1) / RPB), dim3(64), 0,
@@ -936,7 +945,8 @@ int rhs_ws_fields(int physics, int nd) {
 // that does not write it (fu->write_deriv false) follows.
 template <typename T>
 int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx<T>* const* deriv,
-             const cpx<T>* const* forcing, const FusedUpdate<T>* fu, const Ws& ws) {
+             const cpx<T>* const* forcing, const FusedUpdate<T>* fu, const Ws& ws,
+             const T* const* xforcing = nullptr) {
   const int nd = p->ndim;
   PhysLayout L;
   if (phys_layout(prm->physics, nd, &L)) return fail(DDX_ERR_ARG, "unknown physics");
@@ -1018,6 +1028,12 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   for (int q = 0; q < L.nq; ++q) r.in[q] = mixed + q * fs;
   for (int o = 0; o < L.np; ++o) r.out[o] = A + o * fs;
   r.nt = big_nt && DDX_ROW_NT;
+  // an x-space forcing of the velocity (and of T) joins its product: the
+  // product outputs 0 .. nd-1 are the velocity's, output nd is T's
+  if (xforcing) {
+    const int nforce = nd + (prm->physics == DDX_BOUSSINESQ ? 1 : 0);
+    for (int o = 0; o < nforce; ++o) r.xadd[o] = xforcing[o];
+  }
   if ((rc = launch_row(p, r))) return rc;
 
   // forward column passes (single 2D grids with 64-point y axes: the y pass
@@ -1137,7 +1153,8 @@ void comp_coeffs(const ddx_params* prm, int nd, int nstate, double* coeff) {
 template <typename T>
 int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const* y,
               cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2_in,
-              const cpx<T>* const* forcing, int nsteps, double dt, const Ws* fixed_ws = nullptr) {
+              const cpx<T>* const* forcing, int nsteps, double dt, const Ws* fixed_ws = nullptr,
+              const T* const* xforcing = nullptr) {
   const int nd = p->ndim;
   const int ns = nstate_of(prm, nd);
   if (ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
@@ -1181,27 +1198,27 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
       if (integrator == DDX_RK2MID) {
         // time_step.py:285-305
         FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, stage, coeff, vo, dt / 2.0, true};
-        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws))) return rc;
+        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws, xforcing))) return rc;
         FusedUpdate<T> u2{DDX_ETD2RK2, yc, d1c, y, coeff, vo, dt, keep_d2};
-        return rhs_impl<T>(p, prm, stage, keep_d2 ? d2 : nullptr, forcing, &u2, ws);
+        return rhs_impl<T>(p, prm, stage, keep_d2 ? d2 : nullptr, forcing, &u2, ws, xforcing);
       }
       // time_step.py:369-389
       FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, y, coeff, vo, dt, true};
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws, xforcing))) return rc;
       FusedUpdate<T> u2{DDX_ETD2RK1, yc, d1c, y, coeff, vo, dt, keep_d2};
-      return rhs_impl<T>(p, prm, y, keep_d2 ? d2 : nullptr, forcing, &u2, ws);
+      return rhs_impl<T>(p, prm, y, keep_d2 ? d2 : nullptr, forcing, &u2, ws, xforcing);
     }
     if (integrator == DDX_RK2MID) {
       // time_step.py:285-305
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws, xforcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, stage, d1, nullptr, coeff, vo, dt / 2.0))) return rc;
-      if ((rc = rhs_impl<T>(p, prm, stage, d2, forcing, nullptr, ws))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, stage, d2, forcing, nullptr, ws, xforcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK2, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     } else {
       // time_step.py:369-389
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws, xforcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, y, d1, nullptr, coeff, vo, dt))) return rc;
-      if ((rc = rhs_impl<T>(p, prm, y, d2, forcing, nullptr, ws))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d2, forcing, nullptr, ws, xforcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK1, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     }
     return 0;
@@ -1228,7 +1245,9 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
   add(d2v, sizeof(void*) * ns);
   for (int i = 0; i < ns; ++i) {
     const void* f = forcing ? (const void*)forcing[i] : nullptr;
+    const void* xf = xforcing ? (const void*)xforcing[i] : nullptr;
     add(&f, sizeof(f));
+    add(&xf, sizeof(xf));
   }
   add(&p->ws_gen, sizeof(p->ws_gen));
   add(&ws.A, sizeof(void*));
@@ -1818,7 +1837,8 @@ namespace {
 
 int step_dispatch(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
                   void* const* stage, void* const* deriv1, void* const* deriv2,
-                  const void* const* forcing, int nsteps, double dt, const Ws* fixed_ws) {
+                  const void* const* forcing, int nsteps, double dt, const Ws* fixed_ws,
+                  const void* const* xforcing = nullptr) {
   if (check_plan(p) || !prm || !state || !deriv1) return fail(DDX_ERR_ARG, "null argument");
   if (p->dealias == DDX_DEALIAS_NONE) return fail(DDX_ERR_UNSUPPORTED, kNoFusedAliased);
   if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
@@ -1827,10 +1847,12 @@ int step_dispatch(ddx_plan* p, int integrator, const ddx_params* prm, void* cons
   if (p->precision == DDX_F64)
     return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
                              (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2,
-                             (const cpx<double>* const*)forcing, nsteps, dt, fixed_ws);
+                             (const cpx<double>* const*)forcing, nsteps, dt, fixed_ws,
+                             (const double* const*)xforcing);
   return step_impl<float>(p, integrator, prm, (cpx<float>* const*)state, (cpx<float>* const*)st,
                           (cpx<float>* const*)deriv1, (cpx<float>* const*)deriv2,
-                          (const cpx<float>* const*)forcing, nsteps, dt, fixed_ws);
+                          (const cpx<float>* const*)forcing, nsteps, dt, fixed_ws,
+                          (const float* const*)xforcing);
 }
 
 }  // namespace
@@ -1841,6 +1863,22 @@ int ddx_step(ddx_plan* p, int integrator, const ddx_params* prm, void* const* st
              void* const* stage, void* const* deriv1, void* const* deriv2,
              const void* const* forcing, int nsteps, double dt) {
   return step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, nsteps, dt, nullptr);
+}
+
+int ddx_step_xforced(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
+                     void* const* stage, void* const* deriv1, void* const* deriv2,
+                     const void* const* forcing, const void* const* xforcing, int nsteps,
+                     double dt) {
+  if (check_plan(p) || !prm) return fail(DDX_ERR_ARG, "null argument");
+  if (xforcing) {
+    const int ns = nstate_of(prm, p->ndim);
+    if (ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
+    const int nforce = p->ndim + (prm->physics == DDX_BOUSSINESQ ? 1 : 0);
+    for (int i = nforce; i < ns; ++i)
+      if (xforcing[i]) return fail(DDX_ERR_ARG, "x-space forcing applies to the velocity and T only");
+  }
+  return step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, nsteps, dt, nullptr,
+                       xforcing);
 }
 
 int ddx_step_noalloc(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,

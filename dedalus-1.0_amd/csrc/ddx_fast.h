@@ -335,7 +335,7 @@ __device__ __forceinline__ void row_loads(cpx<T> (&la)[PL::E], cpx<T> (&lb)[PL::
     }
 }
 
-template <typename T, int N, int PH, int ND, int V = 0>
+template <typename T, int N, int PH, int ND, int V = 0, bool XF = false>
 __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nrows) {
   using PL = RowPlanV<N, V>;
   constexpr int LANES = PL::LANES, E = PL::E;
@@ -424,6 +424,21 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
     row_physics<T>(PH, ND, a.fpr_inv, qi, oi);
 #pragma unroll
     for (int f = 0; f < NOUT; ++f) o[f][e] = oi[f];
+  }
+  // XF: an x-space forcing joins its product before the r2c (register
+  // qq * RL + rr holds the c2r passes' final position of (lane, qq, rr)); a
+  // variant of its own, so that the kernels without one keep their registers
+  constexpr int RLB = PL::R[PL::NP - 1];
+#pragma unroll
+  for (int f = 0; f < NOUT; ++f) {
+    if (XF && a.xadd[f]) {
+      const T* __restrict__ xa = a.xadd[f] + (live ? r : nrows - 1) * (long)N;
+#pragma unroll
+      for (int qq = 0; qq < E / RLB; ++qq)
+#pragma unroll
+        for (int rr = 0; rr < RLB; ++rr)
+          o[f][qq * RLB + rr] += xa[final_pos<N, PL, false>(lane, qq, rr)];
+    }
   }
 
   cpx<T> twf[PL::NP][E];

@@ -138,6 +138,9 @@ struct RowArgs {
   int mode, physics, ndim;
   T fpr_inv;  // 1 / (4 pi rho0)
   int nt;     // nontemporal output stores (x-row products)
+  // ROW_PHYS: x-space arrays added to product f before its r2c (a forcing
+  // given in x space: the forward transform then carries it), or null
+  const T* xadd[DDX_MAXF];
 };
 
 // x-space products.  Inputs per physics (ndim 3 / 2):
@@ -241,6 +244,9 @@ __global__ void __launch_bounds__(256) row_kernel(const RowArgs<T> a) {
 #pragma unroll
       for (int f = 0; f < 6; ++f) o[f] = T(0);
       row_physics<T>(a.physics, a.ndim, a.fpr_inv, q, o);
+#pragma unroll
+      for (int f = 0; f < 6; ++f)
+        if (f < a.nout && a.xadd[f]) o[f] += a.xadd[f][r * nx + m];
 #pragma unroll
       for (int s = 0; s < 3; ++s)
         if (s < nsout) Y[s * nx + m] = mk(o[2 * s], o[2 * s + 1]);

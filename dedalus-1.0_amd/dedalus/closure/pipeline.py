@@ -348,8 +348,6 @@ class CoarseRun:
                 self.inf = Inference(net, nb, m[0], m[1], fused=fused)
             self.xb = [C.alloc_x() for _ in range(3)]
             self.fx = [C.alloc_x() for _ in range(2)]
-            self.fk = [C.alloc_k() for _ in range(2)]
-            self.wk = C.alloc_k()
             # the coarse forcing in x space joins the network tendency at the
             # unpack, so one forward transform gives their sum
             self.fcx = [C.alloc_x() for _ in range(2)]
@@ -371,7 +369,7 @@ class CoarseRun:
         C = self.C
         forcing = self.forcing
         if self.net is not None:
-            features_x(C, self.state, self.xb, self.wk)
+            features_x(C, self.state, self.xb)
             if self.inf.fused:
                 # features -> network -> tendency + coarse forcing, one launch
                 self.inf.run_fields(self.xb, self.in_scale, self.fx, self.out_scale, add=self.fcx)
@@ -381,8 +379,11 @@ class CoarseRun:
                 out = self.inf.run()
                 unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale,
                        add=[int(b) for b in self.fcx])
-            C.fwd_many(self.fx, self.fk)
-            forcing = self.fk
+            # the x-space sum enters the step's products, whose forward
+            # transforms carry it (no transform of its own)
+            C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,
+                   1, self.dt, xforcing=self.fx)
+            return
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,
                1, self.dt, forcing=forcing)
 

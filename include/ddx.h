@@ -297,6 +297,19 @@ int ddx_step(ddx_plan* plan, int integrator, const ddx_params* params, void* con
              void* const* stage, void* const* deriv1, void* const* deriv2,
              const void* const* forcing, int nsteps, double dt);
 
+/* ddx_step with the forcing given in x space as well as (or instead of) in
+ * k space: xforcing (NULL, or one entry per state component, NULL entries
+ * allowed; velocity components and the Boussinesq T only) holds x-space
+ * arrays of the plan's x layout, added to the right-hand side's x-space
+ * products before their forward transform, so F = forward(f) (1/N,
+ * dealiased) joins every right-hand side as the k-space forcing does.  The
+ * reference's forcing hook (physics.py:524-525, 569-572) returns k-space
+ * arrays; this saves the forward transform of a forcing computed in x space
+ * (the learned closure's tendency), which otherwise runs once per step. */
+int ddx_step_xforced(ddx_plan* plan, int integrator, const ddx_params* prm, void* const* state,
+                     void* const* stage, void* const* deriv1, void* const* deriv2,
+                     const void* const* forcing, const void* const* xforcing, int nsteps,
+                     double dt);
 /* ddx_step that allocates nothing: no hipMalloc, hipFree or other allocating
  * call, and no internal graph capture (the kernels are launched directly on
  * the plan's stream, so a caller may capture the call in a graph of its own).

@@ -710,3 +710,57 @@ def test_bwd_combine(shape, f64):
         want = out['xspace']
         got = plan.download_x(xs[q])
         assert rel_l2(got, want) < (1e-13 if f64 else 2e-6), (q, rel_l2(got, want))
+
+
+@pytest.mark.parametrize("kind,shape,p,integ", [
+    ('hydro', (64, 64), {'nu': 2e-3}, 'RK2mid'),
+    ('hydro', (30, 20), {'nu': 2e-3}, 'RK2trap'),
+    ('hydro', (16, 16, 16), {'nu': 5e-3}, 'RK2mid'),
+    ('boussinesq', (32, 64), {'nu': 1e-3, 'kappa': 1e-3, 'boussinesq_direction': 'y'}, 'RK2mid'),
+    ('mhd', (8, 16, 16), {'nu': 1e-3, 'eta': 1e-3}, 'RK2trap'),
+], ids=str)
+def test_steps_with_x_space_forcing(kind, shape, p, integ):
+    """ddx_step_xforced with an x-space forcing f equals ddx_step with the
+    k-space forcing forward(f) (1/N, dealiased) to rounding: the forcing
+    joins the products before their forward transform instead."""
+    hp = hip()
+    nd = len(shape)
+    L = LENGTHS[nd]
+    case("x-space forcing", kind, shape, integ)
+    plan = hp.Plan(shape, L)
+    kf = kfields(kind, shape, 90, L)
+    prm = params_struct(hp, kind, nd, p)
+    nforce = nd + (1 if kind == 'boussinesq' else 0)
+    xf, kfz = [], []
+    for i in range(nforce):
+        xb = plan.alloc_x()
+        plan.upload_x(xb, 0.3 * random_x(shape, 500 + i))
+        kb = plan.alloc_k()
+        plan.fwd(xb, kb)
+        xf.append(xb)
+        kfz.append(kb)
+    nst = sum(nd if n in ('u', 'B') else 1 for n in FIELDS[kind])
+    xforcing = xf + [None] * (nst - nforce)
+    kforcing = kfz + [None] * (nst - nforce)
+    integ_c = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+    results = []
+    for mode in ('k', 'x'):
+        st = upload_fields(plan, kind, kf)
+        stage = [plan.alloc_k() for _ in st]
+        d1 = [plan.alloc_k() for _ in st]
+        d2 = [plan.alloc_k() for _ in st]
+        if mode == 'k':
+            plan.step(integ_c, prm, st, stage, d1, d2, 3, 4e-3, forcing=kforcing)
+        else:
+            plan.step(integ_c, prm, st, stage, d1, d2, 3, 4e-3, xforcing=xforcing)
+        results.append(download_fields(plan, kind, nd, st))
+    for name in results[0]:
+        for a, b in zip(results[0][name], results[1][name]):
+            assert rel_l2(b, a) < 1e-13, (name, rel_l2(b, a))
+    if kind == 'mhd':
+        # the induction equation has no forcing hook
+        st = upload_fields(plan, kind, kf)
+        bad = xf + [xf[0]] + [None] * (nst - nforce - 1)
+        with pytest.raises(hp.DDXError):
+            plan.step(integ_c, prm, st, [plan.alloc_k() for _ in st], [plan.alloc_k() for _ in st],
+                      None, 1, 4e-3, xforcing=bad)
