@@ -358,6 +358,37 @@ def closure_summary(batches=(1, 64), t_eval=20.0):
     }
 
 
+def copy_run(mib=1024, reps=20):
+    """The HBM rate a plain device copy reaches (ddx_copy of `mib` MiB, one
+    16-byte chunk per thread), timed with HIP events on the plan's stream:
+    the practical ceiling the step's kernels are read against."""
+    import torch
+    from dedalus import _hip
+    _hip.init()
+    n = mib << 20
+    src, dst = _hip.DevBuf(n), _hip.DevBuf(n)
+    plan = _hip.Plan((8, 8), (1.0, 1.0))
+    s = torch.cuda.Stream()
+    plan.set_stream(s.cuda_stream)
+    plan.zero(src)
+    plan.copy(dst, src, n)
+    plan.sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        plan.copy(dst, src, n)
+    e1.record(s)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 2.0 * n / (ms * 1e-3) / 1e9
+    return {"metric": "device copy GB/s (read + write)", "value": round(gbs, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": reps, "higher_is_better": True, "dtype": "u8",
+            "data": "synthetic: zeroed buffer", "config": {"workload": "copy", "MiB": mib},
+            "ms_per_copy": round(ms, 4),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}}
+
+
 def closure_conv_run(batch=256, size=64, reps=20):
     """The closure network's convolutions alone (kernel profiles): the
     preallocated inference pass and one training forward + backward of the
@@ -530,6 +561,9 @@ def main():
 
     if args.workload == "closure_conv":
         print(json.dumps(closure_conv_run(reps=args.steps)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(copy_run(reps=args.steps)), flush=True)
         return
     if args.workload == "ensemble_768":
         from dedalus.ensemble import ENSEMBLE_768
