@@ -459,6 +459,8 @@ class Plan:
     def upload_real_k(self, ref, buf=None):
         """Reference-layout real k array (host) -> compact float64 device
         buffer (a new one unless ``buf`` is given)."""
+        if self.batch:
+            raise ValueError("per-mode k arrays are gathered for plain plans only")
         host = compact_real(ref, self.kref_shape, self.kc_shape, self.cut)
         if buf is None:
             buf = DevBuf(host.nbytes)

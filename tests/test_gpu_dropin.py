@@ -511,3 +511,30 @@ def test_fused_rhs_refuses_undealiased_plans():
     with pytest.raises(_hip.DDXError) as e:
         plan.step(_hip.RK2TRAP, prm, st, None, d1, None, 1, 1e-3)
     assert e.value.code == -3
+
+
+def test_set_dealiasing_moves_the_data():
+    """set_dealiasing on an existing component (representations.py:359-382)
+    rebinds it to a plan of the new rule and keeps its data: k space as set,
+    and a backward transform then dealiases by the new rule, as the
+    reference's backward() does."""
+    m = mods()
+    from dedalus.data_objects.representations import FourierRepresentation
+    case("set_dealiasing migration")
+    shape = (32, 32)
+    with _Dealiasing('2/3'):
+        c = FourierRepresentation(None, shape, (2 * np.pi, 2 * np.pi))
+        oc = npth.Component(shape, (2 * np.pi, 2 * np.pi))
+    kin = random_k(shape, 77)
+    c['kspace'] = kin
+    plan23 = c._plan
+    c.set_dealiasing('2/3 spherical')
+    assert c._plan is not plan23 and c._plan.dealias_rule == 1 and c.nmodes is None
+    assert rel_l2(c['kspace'], kin) < 1e-15
+    x = c['xspace'].copy()
+    oc.set_dealiasing('2/3 spherical')
+    oc['kspace'] = kin
+    assert rel_l2(x, oc['xspace']) < 1e-13
+    assert not np.abs(c['kspace'][c.dealias_mask()]).any()
+    c.set_dealiasing('2/3')
+    assert c._plan.dealias_rule == 0
