@@ -578,20 +578,40 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_tile_kernel(const uint16_t*
 }
 
 // out[j] = sum over chunks of partial[c][j] in a fixed order (double):
-// block = 64 consecutive j x 4 chunk groups (group q sums chunks q, q+4, ...),
-// the four group sums are added in order.
-__global__ void __launch_bounds__(256) sum_chunks_kernel(const float* __restrict__ partial,
-                                                         float* __restrict__ out, int nchunk,
-                                                         long n) {
-  __shared__ double sh[4][64];
+// block = 64 consecutive j x SUM_GROUPS chunk groups (group q sums chunks q,
+// q + SUM_GROUPS, ... with four loads in flight), the group sums are then
+// added in group order.  Sixteen groups keep ~150 k threads loading (a
+// 32 x 32 layer's 9216 weights x 512 chunks is 19 MB).
+constexpr int SUM_GROUPS = 16;
+
+__global__ void __launch_bounds__(64 * SUM_GROUPS) sum_chunks_kernel(const float* __restrict__ partial,
+                                                                     float* __restrict__ out,
+                                                                     int nchunk, long n) {
+  __shared__ double sh[SUM_GROUPS][64];
   const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
   const long j = (long)blockIdx.x * 64 + e;
+  const long jc = j < n ? j : n - 1;   // in-range address for the spare lanes
   double s = 0.0;
-  if (j < n)
-    for (int c = q; c < nchunk; c += 4) s += partial[(long)c * n + j];
+  int c = q;
+  for (; c + 3 * SUM_GROUPS < nchunk; c += 4 * SUM_GROUPS) {
+    const float v0 = partial[(long)c * n + jc];
+    const float v1 = partial[(long)(c + SUM_GROUPS) * n + jc];
+    const float v2 = partial[(long)(c + 2 * SUM_GROUPS) * n + jc];
+    const float v3 = partial[(long)(c + 3 * SUM_GROUPS) * n + jc];
+    s += v0;
+    s += v1;
+    s += v2;
+    s += v3;
+  }
+  for (; c < nchunk; c += SUM_GROUPS) s += partial[(long)c * n + jc];
   sh[q][e] = s;
   __syncthreads();
-  if (q == 0 && j < n) out[j] = (float)(((sh[0][e] + sh[1][e]) + sh[2][e]) + sh[3][e]);
+  if (q == 0 && j < n) {
+    double t = sh[0][e];
+#pragma unroll
+    for (int g = 1; g < SUM_GROUPS; ++g) t += sh[g][e];
+    out[j] = (float)t;
+  }
 }
 
 // Column sums of a [npix][C] fp32 array (bias gradient), two stages with a
@@ -1226,7 +1246,7 @@ int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const v
                          (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W);
     CL_LAUNCHCHK("conv3x3_wgrad tile");
     const long n = (long)Co * 9 * Ci;
-    hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(64 * SUM_GROUPS), 0, s, partial, dw, nch, n);
     CL_LAUNCHCHK("conv3x3_wgrad sum");
     return 0;
   }
@@ -1237,7 +1257,7 @@ int ddx_conv3x3_wgrad(void* stream, int B, int H, int W, int Ci, int Co, const v
                      (const uint16_t*)x, (const uint16_t*)dz, partial, B, H, W, Ci, Co, (int)pch);
   CL_LAUNCHCHK("conv3x3_wgrad");
   const long n = (long)Co * 9 * Ci;
-  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(64 * SUM_GROUPS), 0, s, partial, dw, nch, n);
   CL_LAUNCHCHK("conv3x3_wgrad sum");
   return 0;
 }
@@ -1272,7 +1292,7 @@ int ddx_conv3x3_grads(void* stream, int B, int H, int W, int Ci, int Co, const v
                        B, H, W, dzf, bpartial);
   CL_LAUNCHCHK("conv3x3 grads");
   const long n = (long)Co * 9 * Ci;
-  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, partial, dw, nch, n);
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(n, 64)), dim3(64 * SUM_GROUPS), 0, s, partial, dw, nch, n);
   CL_LAUNCHCHK("conv3x3 grads: weight sum");
   hipLaunchKernelGGL(colsum_final_kernel, dim3(Co), dim3(256), 0, s, bpartial, db, nch, Co);
   CL_LAUNCHCHK("conv3x3 grads: bias sum");

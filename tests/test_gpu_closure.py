@@ -135,19 +135,24 @@ def test_conv_input_gradient(B, H, W, Ci, Co):
 
 
 @pytest.mark.parametrize("B,H,W,Ci,Co", CONV_CASES, ids=str)
-@pytest.mark.parametrize("nchunk", [1, 7, 64])
+@pytest.mark.parametrize("nchunk", [1, 7, 64, 100])
 def test_conv_weight_gradient(B, H, W, Ci, Co, nchunk):
+    """Weight gradient against the float64 one, and the same bits on a second
+    call (the chunk sum adds its partials in a fixed order)."""
     d = dev()
     net = C()
     case("conv weight gradient", B, H, W, Ci, Co, nchunk)
     x, w, _ = random_conv(B, H, W, Ci, Co, 4)
     g = torch.Generator().manual_seed(5)
     dz = torch.randn(B, H, W, Co, generator=g).to(torch.bfloat16)
-    dwp = net.wgrad(x.reshape(-1, Ci).to(d), dz.reshape(-1, Co).to(d), B, H, W, nchunk=nchunk)
+    xd, dzd = x.reshape(-1, Ci).to(d), dz.reshape(-1, Co).to(d)
+    dwp = net.wgrad(xd, dzd, B, H, W, nchunk=nchunk)
     dw = net.wunpack(dwp, Co, Ci)
+    again = net.wgrad(xd, dzd, B, H, W, nchunk=nchunk)
     torch.cuda.synchronize()
     _, ref = R.conv_grads(R.nhwc_to_nchw(x.double()), w.double(), R.nhwc_to_nchw(dz.double()))
     assert rl2(dw, ref) < 1e-5
+    assert torch.equal(dwp, again)
 
 
 def test_colsum_pack_unpack():
