@@ -1094,7 +1094,7 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
     ProfScope prof(p, PC_YEPI,
                    (yin + kvalid * (L.nstate + extra + nforce + upd_arrays * L.nstate)) * sizeof(cpx<T>));
     e.tw_y = (const cpx<T>*)p->tw.at(p->ay.n);
-    const dim3 grid((p->Kxp + YEPI_W - 1) / YEPI_W, p->az.n);
+    const dim3 grid((p->Kxp + YEPI_W - 1) / YEPI_W, p->az.n, YEPI_SPLIT);
     hipLaunchKernelGGL((yepi64_kernel<T>), grid, dim3(64 * YEPI_WAVES), 0, p->stream, e);
     LAUNCHCHK();
     return 0;

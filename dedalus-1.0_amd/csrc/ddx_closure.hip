@@ -1455,9 +1455,16 @@ int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
   const bool small_ok = H % 8 == 0 && W % 16 == 0;
   if (nl == 5 && hid == 32) {
     if (big >= 256 || (big > 0 && !small_ok)) return launch_net<5, 32, 32, 16, 512>(a, s);
-    // a few images (a single coarse run): 8 x 8 tiles, each layer's region
-    // at most one fragment pair per wavefront, so the five layers are five
-    // short dependent phases rather than rounds of pairs
+    // a few images: small tiles, whose layers are a few short dependent
+    // phases each (one 64^2 image, a single coarse run: 4 x 4 tiles, one
+    // per CU, 11.7 us per pass against 14.7 us with 8 x 8 tiles, though the
+    // executed MFMA work grows from 3.0x to 5.5x the network's: the pass
+    // is a chain of latencies, not of arithmetic)
+    if ((long)B * (H / 4) * (W / 4) <= 256 && H % 4 == 0 && W % 4 == 0)
+      return launch_net<5, 32, 4, 4, 256>(a, s);
+    // then 8 x 8 tiles: each layer's region at most one fragment pair per
+    // wavefront, so the five layers are five short phases rather than rounds
+    // of pairs
     const long small = small_ok ? (long)B * (H / 8) * (W / 16) : 0;
     if (small < 128 && H % 8 == 0 && W % 8 == 0) return launch_net<5, 32, 8, 8, 512>(a, s);
     if (small_ok) return launch_net<5, 32, 16, 8, 256>(a, s);

@@ -788,15 +788,22 @@ col_fft_fast(const ColArgs<T> a) {
 
 constexpr int YEPI_W = 8;      // kx columns per workgroup
 constexpr int YEPI_WAVES = 4;  // wavefront f transforms product field f
+constexpr int YEPI_SPLIT = 2;  // workgroups per column group (blockIdx.z: share of the ky rows)
 
-// One workgroup: YEPI_W kx columns of one member, every product field (2 or
-// 3).  Wavefront f runs the y transform of field f (the column plan of 64:
-// 8 lanes x 8 columns), so the fields are transformed side by side; a
-// wavefront without a field repeats field 0 and discards it, which keeps the
-// workgroup barriers uniform.  The kept ky modes go to LDS, and then all
-// wavefronts run the epilogue (epi_mode) one mode per thread.  This saves
-// the compact products' round trip through HBM and, on the 64^2 grids where
-// launch latency sets the time, one launch per right-hand side.
+// YEPI_SPLIT workgroups per YEPI_W kx columns of one member.  Each transforms
+// every product field (2 or 3) of the columns: wavefront f runs the y
+// transform of field f (the column plan of 64: 8 lanes x 8 columns), so the
+// fields are transformed side by side; a wavefront without a field repeats
+// field 0 and discards it, which keeps the workgroup barriers uniform.  The
+// kept ky modes go to LDS, and then the workgroup runs the epilogue
+// (epi_mode) of its share of the ky rows, one mode per thread.  The mode's
+// other reads (the sweep's y and F1, the Boussinesq T and buoyancy velocity,
+// the wavenumbers) are issued before the transform, so that their latency
+// overlaps it; the shares make one mode per thread enough (the transform is
+// repeated per share: at 64^2 launch latency and dependent memory round
+// trips set the time, not the arithmetic).  This saves the compact
+// products' round trip through HBM and, on the 64^2 grids, one launch per
+// right-hand side.
 // a.U[f]: the x-row products, layout (members, 64, Kxp).
 template <typename T>
 __global__ void __launch_bounds__(64 * YEPI_WAVES) yepi64_kernel(const EpiArgs<T> a) {
@@ -807,6 +814,7 @@ __global__ void __launch_bounds__(64 * YEPI_WAVES) yepi64_kernel(const EpiArgs<T
   constexpr int R0 = PL::R[0], RL = PL::R[PL::NP - 1];
   constexpr int BUF = exch_size<N, PL>();
   constexpr int KYMAX = 2 * ((N + 2) / 3) + 1;   // bound on the kept rows
+  static_assert((KYMAX + YEPI_SPLIT - 1) / YEPI_SPLIT * W <= 64 * YEPI_WAVES, "one mode per thread");
   constexpr bool POW = sizeof(T) == 8;
   __shared__ cpx<T> lds[YEPI_WAVES][BUF * W];
   __shared__ cpx<T> stage[3][KYMAX * W];
@@ -821,6 +829,39 @@ __global__ void __launch_bounds__(64 * YEPI_WAVES) yepi64_kernel(const EpiArgs<T
   const int nf = 2 + ((bouss || mhd) ? 1 : 0);
   const int nvar = 2 + (bouss ? 1 : (mhd ? 2 : 0));
   const int f = wave < nf ? wave : 0;
+  const cpx<T> z = mk(T(0), T(0));
+
+  // this thread's mode: row j (of this workgroup's share) and column cl
+  const int KH = (g.Ky + YEPI_SPLIT - 1) / YEPI_SPLIT;
+  const int jj = threadIdx.x / W, cc = threadIdx.x - jj * W;
+  const int j = blockIdx.z * KH + jj;
+  const int cl = blockIdx.x * W + cc;
+  const bool mine = jj < KH && j < g.Ky && cl < g.Kxp;
+  const bool live = mine && cl < g.Kx;
+  const long idx = ((long)j * g.Kz + slab) * g.Kxp + cl;
+  cpx<T> yv[6], fv[6], tval = z, bval = z;
+#pragma unroll
+  for (int v = 0; v < 6; ++v) yv[v] = fv[v] = z;
+  double kx = 0.0, ky = 0.0, kz = 0.0;
+  if (live) {
+    kx = g.kx[cl];
+    ky = g.ky[j];
+    kz = g.kz[slab];
+    if (bouss) {
+      tval = a.state[2][idx];
+      bval = (a.bdir == 0 ? a.state[0] : a.state[1])[idx];
+    }
+    if (a.upd_kind >= 0) {
+      const bool hasf = a.upd_kind >= UPD_ETD2RK1;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (v < nvar) {
+          yv[v] = a.upd_y[v][idx];
+          if (hasf) fv[v] = a.upd_f1[v][idx];
+        }
+    }
+  }
+
   cpx<T> twc[PL::NP][E];
   fill_twiddles<T, N, PL, false, POW>(twc, lane, a.tw_y);
   cpx<T> v[E];
@@ -837,30 +878,19 @@ __global__ void __launch_bounds__(64 * YEPI_WAVES) yepi64_kernel(const EpiArgs<T
 #pragma unroll
       for (int rr = 0; rr < RL; ++rr) {
         const int m = final_pos<N, PL, false>(lane, qq, rr);
-        const int j = m <= cy ? m : (m >= N - cy ? m - (N - g.Ky) : -1);
-        if (j >= 0) stage[wave][j * W + c] = v[qq * RL + rr];
+        const int jm = m <= cy ? m : (m >= N - cy ? m - (N - g.Ky) : -1);
+        if (jm >= 0) stage[wave][jm * W + c] = v[qq * RL + rr];
       }
   }
   __syncthreads();
-  const cpx<T> z = mk(T(0), T(0));
-  for (int i = threadIdx.x; i < g.Ky * W; i += blockDim.x) {
-    const int j = i / W, cc = i - j * W;
-    const int cl = blockIdx.x * W + cc;
-    if (cl >= g.Kxp) continue;
-    const long idx = ((long)j * g.Kz + slab) * g.Kxp + cl;
-    if (cl >= g.Kx) {
-      epi_pad(a, idx, nvar);
-      continue;
-    }
-    cpx<T> tval = z, bval = z;
-    if (bouss) {
-      tval = a.state[2][idx];
-      bval = (a.bdir == 0 ? a.state[0] : a.state[1])[idx];
-    }
-    const cpx<T> e0 = nf > 2 ? stage[2][i] : z;
-    epi_mode(a, idx, g.kx[cl], g.ky[j], g.kz[slab], stage[0][i], stage[1][i], z, e0, z, z, tval,
-             bval);
+  if (!mine) return;
+  if (!live) {
+    epi_pad(a, idx, nvar);
+    return;
   }
+  const int i = j * W + cc;
+  const cpx<T> e0 = nf > 2 ? stage[2][i] : z;
+  epi_mode(a, idx, kx, ky, kz, stage[0][i], stage[1][i], z, e0, z, z, tval, bval, yv, fv);
 }
 
 }  // namespace ddx
