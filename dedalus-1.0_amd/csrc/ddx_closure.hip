@@ -906,7 +906,17 @@ struct NetPlan {
       if ((l + 1) % 2 == b && out_bytes(l) > m) m = out_bytes(l);
     return (m + 15) / 16 * 16;
   }
-  static constexpr int lds_bytes() { return buf_bytes(0) + buf_bytes(1); }
+  // the current layer's weights, staged as MFMA A operands (NetStage)
+  static constexpr int wbuf_bytes() {
+    int m = 0;
+    for (int l = 0; l < NL; ++l) {
+      const int ci = l == 0 ? 16 : HID, co = l == NL - 1 ? 16 : HID;
+      const int b = 9 * (co / 16) * 64 * (ci == 32 ? 16 : 8);
+      if (b > m) m = b;
+    }
+    return m;
+  }
+  static constexpr int lds_bytes() { return buf_bytes(0) + buf_bytes(1) + wbuf_bytes(); }
 };
 
 __device__ inline bf16x8 gld8(const uint16_t* p) {
@@ -922,32 +932,52 @@ __device__ inline int wrapm(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
-// A layer's weights and bias in registers: the weights as MFMA A operands
-// (lane (m, kq) holds output channel j*16 + m, input channels kq*8.. (or
-// kq*4..) of tap t), the bias of the four output channels a lane's D values
-// hold.  Loaded one layer ahead, so the loads run under the previous layer's
-// arithmetic instead of stalling each layer's start.
-template <int CI, int CO>
-struct NetWeights {
-  static constexpr int NCO = CO / 16;
-  typedef typename std::conditional<CI == 32, bf16x8, bf16x4>::type wvec;
-  wvec w[9][NCO];
+// A layer's weights staged through registers into LDS, and its bias.  The
+// LDS copy holds the MFMA A operands in lane order: block (t, j) (tap t,
+// output channels 16 j ..) is 64 consecutive chunks, chunk l = output channel
+// 16 j + l % 16, input channels (l / 16) KQ .. (KQ = 8 of 32 input channels,
+// 4 of 16), so a wavefront's operand read of one block is one contiguous run
+// of LDS.  The workgroup's threads share the chunks: the layer is read from
+// global memory once per tile, not once per wavefront.  load() runs a layer
+// ahead, under the previous layer's arithmetic; the bias stays in registers
+// (a lane's four output channels of each 16).
+__device__ inline bf16x8 lds16(const uint16_t* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const u32x4 u = {v.x, v.y, v.z, v.w};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+template <int CI, int CO, int NT>
+struct NetStage {
+  static constexpr int NCO = CO / 16, KQ = CI == 32 ? 8 : 4;
+  static constexpr int NCH = 9 * NCO * 64, R = (NCH + NT - 1) / NT;
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  typedef typename std::conditional<CI == 32, u32x4, u32x2>::type chunk;
+  chunk c[R];
   float b[NCO][4];
   __device__ __forceinline__ void load(const uint16_t* w_l, const float* b_l) {
-    const int lane = threadIdx.x & 63;
-    const int m = lane & 15, kq = lane >> 4;
+    // unconditional loads (threads past the last chunk repeat it; store()
+    // skips them), which keeps the chunks in registers
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-      for (int j = 0; j < NCO; ++j) {
-        const uint16_t* q = w_l + ((j * 16 + m) * 9 + t) * CI;
-        if constexpr (CI == 32) w[t][j] = gld8(q + 8 * kq);
-        else w[t][j] = gld4(q + 4 * kq);
-      }
+    for (int r = 0; r < R; ++r) {
+      int q = (int)threadIdx.x + r * NT;
+      q = q < NCH ? q : NCH - 1;
+      const int blk = q >> 6, l = q & 63;
+      const int t = blk / NCO, j = blk - t * NCO;
+      c[r] = *reinterpret_cast<const chunk*>(w_l + ((j * 16 + (l & 15)) * 9 + t) * CI + (l >> 4) * KQ);
+    }
+    const int kq = (threadIdx.x & 63) >> 4;
 #pragma unroll
     for (int j = 0; j < NCO; ++j)
 #pragma unroll
       for (int i = 0; i < 4; ++i) b[j][i] = b_l[j * 16 + 4 * kq + i];
+  }
+  __device__ __forceinline__ void store(uint16_t* wl) const {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = (int)threadIdx.x + r * NT;
+      if (q < NCH) *reinterpret_cast<chunk*>(wl + q * KQ) = c[r];
+    }
   }
 };
 
@@ -956,15 +986,13 @@ struct NetWeights {
 // and store nothing).  in: (w + 2) x (h + 2) pixels with stride psi;
 // LAST layers write the tile's pixels to global memory.
 template <int CI, int CO, bool LAST, int NT>
-__device__ __forceinline__ void net_layer(const NetArgs& a, const NetWeights<CI, CO>& wr,
+__device__ __forceinline__ void net_layer(const NetArgs& a, const uint16_t* wl, const float (&bv)[CO / 16][4],
                                           const uint16_t* in, int psi, uint16_t* outb, int pso,
                                           int w, int h, int b, int y0, int x0) {
   constexpr int NCO = CO / 16;
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  const auto& wv = wr.w;
-  const auto& bv = wr.b;
   const int npx = w * h, nfrag = (npx + 15) / 16;
   const int wi = w + 2;
   const int npairs = (nfrag + 1) / 2;
@@ -991,19 +1019,27 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const NetWeights<CI,
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int off = (t / 3) * wi + (t % 3);
+      if constexpr (CI == 32) {
+        bf16x8 wv[NCO];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const uint16_t* q = in + (base[u] + off) * psi;
-        if constexpr (CI == 32) {
-          const bf16x8 xv = lds8(q + 8 * kq);
+        for (int j = 0; j < NCO; ++j) wv[j] = lds16(wl + ((t * NCO + j) * 64 + lane) * 8);
 #pragma unroll
-          for (int j = 0; j < NCO; ++j)
-            acc[u][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[t][j], xv, acc[u][j], 0, 0, 0);
-        } else {
-          const bf16x4 xv = lds4(q + 4 * kq);
+        for (int u = 0; u < 2; ++u) {
+          const bf16x8 xv = lds8(in + (base[u] + off) * psi + 8 * kq);
 #pragma unroll
           for (int j = 0; j < NCO; ++j)
-            acc[u][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv[t][j], xv, acc[u][j], 0, 0, 0);
+            acc[u][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[j], xv, acc[u][j], 0, 0, 0);
+        }
+      } else {
+        bf16x4 wv[NCO];
+#pragma unroll
+        for (int j = 0; j < NCO; ++j) wv[j] = lds4(wl + ((t * NCO + j) * 64 + lane) * 4);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const bf16x4 xv = lds4(in + (base[u] + off) * psi + 4 * kq);
+#pragma unroll
+          for (int j = 0; j < NCO; ++j)
+            acc[u][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv[j], xv, acc[u][j], 0, 0, 0);
         }
       }
     }
@@ -1069,34 +1105,38 @@ __device__ __forceinline__ void net_layer(const NetArgs& a, const NetWeights<CI,
   if (pendB) finish(accB, ppB, vB);
 }
 
-template <int L, int NL, int HID>
+template <int L, int NL, int HID, int NT>
 struct NetLayerShape {
   static constexpr int CI = L == 0 ? 16 : HID;
   static constexpr int CO = L == NL - 1 ? 16 : HID;
-  typedef NetWeights<CI, CO> W;
+  typedef NetStage<CI, CO, NT> W;
 };
 
-// Layer L with its weights already in registers: the next layer's weights
-// are loaded first, then layer L runs, then the next layer after a barrier.
+// Layer L with its weights in LDS (wl) and its bias in cur: the next
+// layer's weights are loaded into registers first, then layer L runs; after
+// a barrier (every wavefront done with layer L's input and weights) the next
+// weights go to LDS, and after a second one the next layer runs.
 template <int L, int NL, int HID, int TX, int TY, int NT>
-__device__ __forceinline__ void net_layers(const NetArgs& a, uint16_t* buf0, uint16_t* buf1, int b,
-                                           int y0, int x0,
-                                           const typename NetLayerShape<L, NL, HID>::W& cur) {
+__device__ __forceinline__ void net_layers(const NetArgs& a, uint16_t* buf0, uint16_t* buf1, uint16_t* wl,
+                                           int b, int y0, int x0,
+                                           const typename NetLayerShape<L, NL, HID, NT>::W& cur) {
   using P = NetPlan<NL, HID, TX, TY>;
-  using S = NetLayerShape<L, NL, HID>;
+  using S = NetLayerShape<L, NL, HID, NT>;
   constexpr int g = NL - 1 - L;  // output halo
   const uint16_t* in = (L % 2 == 0) ? buf0 : buf1;
   uint16_t* outb = (L % 2 == 0) ? buf1 : buf0;
   if constexpr (L < NL - 1) {
-    typename NetLayerShape<L + 1, NL, HID>::W nxt;
+    typename NetLayerShape<L + 1, NL, HID, NT>::W nxt;
     nxt.load(a.w[L + 1], a.bias[L + 1]);
-    net_layer<S::CI, S::CO, false, NT>(a, cur, in, L == 0 ? P::PSI : P::PSH, outb, P::PSH,
+    net_layer<S::CI, S::CO, false, NT>(a, wl, cur.b, in, L == 0 ? P::PSI : P::PSH, outb, P::PSH,
                                        TX + 2 * g, TY + 2 * g, b, y0, x0);
     lds_barrier();
-    net_layers<L + 1, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0, nxt);
+    nxt.store(wl);
+    lds_barrier();
+    net_layers<L + 1, NL, HID, TX, TY, NT>(a, buf0, buf1, wl, b, y0, x0, nxt);
   } else {
-    net_layer<S::CI, S::CO, true, NT>(a, cur, in, P::PSH, outb, P::PSH, TX + 2 * g, TY + 2 * g, b,
-                                      y0, x0);
+    net_layer<S::CI, S::CO, true, NT>(a, wl, cur.b, in, P::PSH, outb, P::PSH, TX + 2 * g, TY + 2 * g,
+                                      b, y0, x0);
   }
 }
 
@@ -1106,6 +1146,7 @@ __global__ void __launch_bounds__(NT) conv_net_kernel(NetArgs a) {
   extern __shared__ __align__(16) uint16_t netlds[];
   uint16_t* buf0 = netlds;
   uint16_t* buf1 = netlds + P::buf_bytes(0) / 2;
+  uint16_t* wl = netlds + (P::buf_bytes(0) + P::buf_bytes(1)) / 2;
   const int tiles_x = a.W / TX, tiles_y = a.H / TY;
   int rest = blockIdx.x;
   const int tx = rest % tiles_x;
@@ -1114,7 +1155,7 @@ __global__ void __launch_bounds__(NT) conv_net_kernel(NetArgs a) {
   const int b = rest / tiles_y;
   const int x0 = tx * TX, y0 = ty * TY;
   // the first layer's weights load while the input is staged
-  typename NetLayerShape<0, NL, HID>::W w0;
+  typename NetLayerShape<0, NL, HID, NT>::W w0;
   w0.load(a.w[0], a.bias[0]);
   // stage the input region with a halo of NL pixels (periodic wrap)
   constexpr int RW = TX + 2 * NL, RH = TY + 2 * NL;
@@ -1148,8 +1189,9 @@ __global__ void __launch_bounds__(NT) conv_net_kernel(NetArgs a) {
     d[2] = make_uint2(q1.x, q1.y);
     d[3] = make_uint2(q1.z, q1.w);
   }
+  w0.store(wl);
   lds_barrier();
-  net_layers<0, NL, HID, TX, TY, NT>(a, buf0, buf1, b, y0, x0, w0);
+  net_layers<0, NL, HID, TX, TY, NT>(a, buf0, buf1, wl, b, y0, x0, w0);
 }
 
 template <int NL, int HID, int TX, int TY, int NT>
@@ -1457,7 +1499,7 @@ int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
     if (big >= 256 || (big > 0 && !small_ok)) return launch_net<5, 32, 32, 16, 512>(a, s);
     // a few images: small tiles, whose layers are a few short dependent
     // phases each (one 64^2 image, a single coarse run: 4 x 4 tiles, one
-    // per CU, 11.7 us per pass against 14.7 us with 8 x 8 tiles, though the
+    // per CU, 9.4 us per pass against 10.0 us with 8 x 8 tiles, though the
     // executed MFMA work grows from 3.0x to 5.5x the network's: the pass
     // is a chain of latencies, not of arithmetic)
     if ((long)B * (H / 4) * (W / 4) <= 256 && H % 4 == 0 && W % 4 == 0)
