@@ -1496,7 +1496,7 @@ int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
   const long big = (H % 16 == 0 && W % 32 == 0) ? (long)B * (H / 16) * (W / 32) : 0;
   const bool small_ok = H % 8 == 0 && W % 16 == 0;
   if (nl == 5 && hid == 32) {
-    if (big >= 256 || (big > 0 && !small_ok)) return launch_net<5, 32, 32, 16, 512>(a, s);
+    if (big >= 256 || (big > 0 && !small_ok)) return launch_net<5, 32, 32, 16, 768>(a, s);
     // a few images: small tiles, whose layers are a few short dependent
     // phases each (one 64^2 image, a single coarse run: 4 x 4 tiles, one
     // per CU, 9.4 us per pass against 10.0 us with 8 x 8 tiles, though the
