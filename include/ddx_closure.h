@@ -118,9 +118,10 @@ int ddx_box_filter(void* stream, int ny, int nx, int factor, int f64, const void
  * and W multiples of 8) and depth 3 with 16 (H a multiple of 8, W of 16),
  * inputs and outputs of at most 16 channels; otherwise DDX_ERR_UNSUPPORTED
  * and the layer-by-layer calls apply.  The tile size follows the image
- * count: 32 x 16 output tiles when they alone fill the chip, 8 x 8 for a few
- * images (one coarse run), 16 x 8 in between.  in_scale, out_scale, add may
- * be NULL (1, 1, nothing). */
+ * count: 32 x 16 output tiles when they alone fill the chip, 4 x 4 when at
+ * most 256 of them cover the images (one 64^2 coarse run), 8 x 8 for a few
+ * images, 16 x 8 in between.  in_scale, out_scale, add may be NULL (1, 1,
+ * nothing). */
 int ddx_conv_net(void* stream, int B, int H, int W, int nl, const int* chans,
                  const void* const* w, const float* const* bias, const void* x, float* out,
                  const void* const* src, const float* in_scale, void* const* dst,
