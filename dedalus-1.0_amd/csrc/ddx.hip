@@ -511,6 +511,15 @@ void geom_fz(const ddx_plan* p, ColArgs<T>& a) {
 
 template <typename T, int N, int PH, int ND>
 void row_fast_go(ddx_plan* p, const RowArgs<T>& a, long nrows) {
+  if constexpr (ND == 2) {
+    if (a.xin[0]) {
+      // inputs given in x space (rhs_impl checks that this kernel applies)
+      constexpr int RPB = 64 / RowPlan<N>::LANES;
+      hipLaunchKernelGGL((row_phys_fast<T, N, PH, ND, 0, true, true>), dim3((nrows + RPB - 1) / RPB),
+                         dim3(64), 0, p->stream, a, nrows);
+      return;
+    }
+  }
   bool xf = false;
   for (int f = 0; f < a.nout; ++f) xf = xf || a.xadd[f] != nullptr;
   if (xf) {
@@ -943,10 +952,21 @@ int rhs_ws_fields(int physics, int nd) {
 // The right-hand side on the workspace ws (ws.A, ws.B hold rhs_ws_fields
 // arrays of ws_field_stride each).  deriv may be null when a fused sweep
 // that does not write it (fu->write_deriv false) follows.
+// xin (2D): the x-space inputs of the x-row kernel (the recipe fields of
+// phys_layout, as ddx_bwd_combine writes them), which then replace the
+// backward column pass; used where the size-specialised x-row kernel applies
+// to a cube-rule plan, otherwise ignored (the pass runs).
+template <typename T>
+bool xin_applies(const ddx_plan* p, int nq) {
+  return p->ndim == 2 && !p->generic_only && p->dealias == DDX_DEALIAS_23 && nq <= 12 &&
+         p->Kx <= p->ax.n / 3 + 1 &&
+         (p->ax.n == 64 || p->ax.n == 128 || p->ax.n == 256 || p->ax.n == 512);
+}
+
 template <typename T>
 int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx<T>* const* deriv,
              const cpx<T>* const* forcing, const FusedUpdate<T>* fu, const Ws& ws,
-             const T* const* xforcing = nullptr) {
+             const T* const* xforcing = nullptr, const T* const* xin = nullptr) {
   const int nd = p->ndim;
   PhysLayout L;
   if (phys_layout(prm->physics, nd, &L)) return fail(DDX_ERR_ARG, "unknown physics");
@@ -987,7 +1007,10 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   // forward z pass storing so was no faster, and at 128^3, where the
   // intermediates fit the last-level cache, it was slower.
   const bool big_nt = DDX_COL_NT && p->zfft && (long)p->az.n * p->ay.n * p->Kxp >= (1L << 22);
-  if (p->zfft && nzs < L.nq) {
+  const bool use_xin = xin && xin_applies<T>(p, L.nq);
+  if (use_xin) {
+    // the caller's x-space inputs replace the backward pass
+  } else if (p->zfft && nzs < L.nq) {
     geom_bz(p, c);
     for (int j = 0; j < nzs; ++j) { c.rec[j] = zrec[j]; c.out[j] = A + j * fs; }
     c.nt = big_nt;
@@ -1026,6 +1049,8 @@ int rhs_impl(ddx_plan* p, const ddx_params* prm, const cpx<T>* const* state, cpx
   r.nout = L.np;
   r.fpr_inv = T(1.0 / (4.0 * M_PI * (prm->physics == DDX_MHD ? prm->rho0 : 1.0)));
   for (int q = 0; q < L.nq; ++q) r.in[q] = mixed + q * fs;
+  if (use_xin)
+    for (int q = 0; q < L.nq; ++q) r.xin[q] = xin[q];
   for (int o = 0; o < L.np; ++o) r.out[o] = A + o * fs;
   r.nt = big_nt && DDX_ROW_NT;
   // an x-space forcing of the velocity (and of T) joins its product: the
@@ -1154,9 +1179,11 @@ template <typename T>
 int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const* y,
               cpx<T>* const* stage, cpx<T>* const* d1, cpx<T>* const* d2_in,
               const cpx<T>* const* forcing, int nsteps, double dt, const Ws* fixed_ws = nullptr,
-              const T* const* xforcing = nullptr) {
+              const T* const* xforcing = nullptr, const T* const* xin = nullptr) {
   const int nd = p->ndim;
   const int ns = nstate_of(prm, nd);
+  // xin describes the state before the first step only
+  if (xin && nsteps != 1) return fail(DDX_ERR_ARG, "x-space inputs apply to a single step");
   if (ns < 0) return fail(DDX_ERR_ARG, "unknown physics");
   if (integrator != DDX_RK2MID && integrator != DDX_RK2TRAP)
     return fail(DDX_ERR_ARG, "unknown integrator");
@@ -1198,25 +1225,25 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
       if (integrator == DDX_RK2MID) {
         // time_step.py:285-305
         FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, stage, coeff, vo, dt / 2.0, true};
-        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws, xforcing))) return rc;
+        if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws, xforcing, xin))) return rc;
         FusedUpdate<T> u2{DDX_ETD2RK2, yc, d1c, y, coeff, vo, dt, keep_d2};
         return rhs_impl<T>(p, prm, stage, keep_d2 ? d2 : nullptr, forcing, &u2, ws, xforcing);
       }
       // time_step.py:369-389
       FusedUpdate<T> u1{DDX_ETD1, yc, nullptr, y, coeff, vo, dt, true};
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws, xforcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, &u1, ws, xforcing, xin))) return rc;
       FusedUpdate<T> u2{DDX_ETD2RK1, yc, d1c, y, coeff, vo, dt, keep_d2};
       return rhs_impl<T>(p, prm, y, keep_d2 ? d2 : nullptr, forcing, &u2, ws, xforcing);
     }
     if (integrator == DDX_RK2MID) {
       // time_step.py:285-305
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws, xforcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws, xforcing, xin))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, stage, d1, nullptr, coeff, vo, dt / 2.0))) return rc;
       if ((rc = rhs_impl<T>(p, prm, stage, d2, forcing, nullptr, ws, xforcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK2, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
     } else {
       // time_step.py:369-389
-      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws, xforcing))) return rc;
+      if ((rc = rhs_impl<T>(p, prm, y, d1, forcing, nullptr, ws, xforcing, xin))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD1, ns, y, y, d1, nullptr, coeff, vo, dt))) return rc;
       if ((rc = rhs_impl<T>(p, prm, y, d2, forcing, nullptr, ws, xforcing))) return rc;
       if ((rc = update_impl<T>(p, DDX_ETD2RK1, ns, y, y, d1, d2, coeff, vo, dt))) return rc;
@@ -1248,6 +1275,14 @@ int step_impl(ddx_plan* p, int integrator, const ddx_params* prm, cpx<T>* const*
     const void* xf = xforcing ? (const void*)xforcing[i] : nullptr;
     add(&f, sizeof(f));
     add(&xf, sizeof(xf));
+  }
+  {
+    PhysLayout L;
+    const int nq = phys_layout(prm->physics, nd, &L) ? 0 : L.nq;
+    for (int q = 0; q < nq; ++q) {
+      const void* xi = xin ? (const void*)xin[q] : nullptr;
+      add(&xi, sizeof(xi));
+    }
   }
   add(&p->ws_gen, sizeof(p->ws_gen));
   add(&ws.A, sizeof(void*));
@@ -1841,7 +1876,7 @@ namespace {
 int step_dispatch(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
                   void* const* stage, void* const* deriv1, void* const* deriv2,
                   const void* const* forcing, int nsteps, double dt, const Ws* fixed_ws,
-                  const void* const* xforcing = nullptr) {
+                  const void* const* xforcing = nullptr, const void* const* xin = nullptr) {
   if (check_plan(p) || !prm || !state || !deriv1) return fail(DDX_ERR_ARG, "null argument");
   if (p->dealias == DDX_DEALIAS_NONE) return fail(DDX_ERR_UNSUPPORTED, kNoFusedAliased);
   if (integrator == DDX_RK2MID && !stage) return fail(DDX_ERR_ARG, "RK2mid needs stage buffers");
@@ -1851,11 +1886,11 @@ int step_dispatch(ddx_plan* p, int integrator, const ddx_params* prm, void* cons
     return step_impl<double>(p, integrator, prm, (cpx<double>* const*)state, (cpx<double>* const*)st,
                              (cpx<double>* const*)deriv1, (cpx<double>* const*)deriv2,
                              (const cpx<double>* const*)forcing, nsteps, dt, fixed_ws,
-                             (const double* const*)xforcing);
+                             (const double* const*)xforcing, (const double* const*)xin);
   return step_impl<float>(p, integrator, prm, (cpx<float>* const*)state, (cpx<float>* const*)st,
                           (cpx<float>* const*)deriv1, (cpx<float>* const*)deriv2,
                           (const cpx<float>* const*)forcing, nsteps, dt, fixed_ws,
-                          (const float* const*)xforcing);
+                          (const float* const*)xforcing, (const float* const*)xin);
 }
 
 }  // namespace
@@ -1882,6 +1917,26 @@ int ddx_step_xforced(ddx_plan* p, int integrator, const ddx_params* prm, void* c
   }
   return step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, nsteps, dt, nullptr,
                        xforcing);
+}
+
+int ddx_step_xin(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,
+                 void* const* stage, void* const* deriv1, void* const* deriv2,
+                 const void* const* forcing, const void* const* xforcing, const void* const* xin,
+                 double dt) {
+  if (check_plan(p) || !prm || !xin) return fail(DDX_ERR_ARG, "null argument");
+  PhysLayout L;
+  if (phys_layout(prm->physics, p->ndim, &L)) return fail(DDX_ERR_ARG, "unknown physics");
+  if (p->ndim != 2) return fail(DDX_ERR_UNSUPPORTED, "x-space step inputs are for 2D grids");
+  for (int q = 0; q < L.nq; ++q)
+    if (!xin[q]) return fail(DDX_ERR_ARG, "null x-space input");
+  if (xforcing) {
+    const int ns = L.nstate;
+    const int nforce = p->ndim + (prm->physics == DDX_BOUSSINESQ ? 1 : 0);
+    for (int i = nforce; i < ns; ++i)
+      if (xforcing[i]) return fail(DDX_ERR_ARG, "x-space forcing applies to the velocity and T only");
+  }
+  return step_dispatch(p, integrator, prm, state, stage, deriv1, deriv2, forcing, 1, dt, nullptr,
+                       xforcing, xin);
 }
 
 int ddx_step_noalloc(ddx_plan* p, int integrator, const ddx_params* prm, void* const* state,

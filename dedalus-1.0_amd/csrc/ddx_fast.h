@@ -335,7 +335,10 @@ __device__ __forceinline__ void row_loads(cpx<T> (&la)[PL::E], cpx<T> (&lb)[PL::
     }
 }
 
-template <typename T, int N, int PH, int ND, int V = 0, bool XF = false>
+// XIN: the inputs arrive in x space (a.xin[f], rows of N reals, the values
+// a backward transform of the same fields gives), so the c2r passes are
+// skipped; a variant of its own, with XF.
+template <typename T, int N, int PH, int ND, int V = 0, bool XF = false, bool XIN = false>
 __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nrows) {
   using PL = RowPlanV<N, V>;
   constexpr int LANES = PL::LANES, E = PL::E;
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
   // FP64 rows of 12 fields park the x-space values of the first two input
   // pairs in LDS until the products (8 KB per row at n = 256), which keeps
   // the kernel within the registers of 3 wavefronts per SIMD
-  constexpr int NPARK = (NIN >= 12 && sizeof(T) == 8) ? 4 : 0;
+  constexpr int NPARK = (NIN >= 12 && sizeof(T) == 8 && !XIN) ? 4 : 0;
   __shared__ cpx<T> lds[RPB][BUF];
   __shared__ T park[RPB][NPARK > 0 ? NPARK : 1][NPARK > 0 ? N : 1];
   const int team = threadIdx.x / LANES, lane = threadIdx.x % LANES;
@@ -371,6 +374,18 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
   // transform of pair p (two pairs in flight per wavefront).
   const long rl = live ? r : nrows - 1;
   T q[NIN][E];
+  constexpr int RLB = PL::R[PL::NP - 1];
+  if constexpr (XIN) {
+    // register qq * RLB + rr holds the c2r passes' final position of (lane, qq, rr)
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) {
+      const T* __restrict__ xr = a.xin[f] + rl * (long)N;
+#pragma unroll
+      for (int qq = 0; qq < E / RLB; ++qq)
+#pragma unroll
+        for (int rr = 0; rr < RLB; ++rr) q[f][qq * RLB + rr] = xr[final_pos<N, PL, false>(lane, qq, rr)];
+    }
+  } else {
   cpx<T> la[2][E], lb[2][E];
   row_loads<T, N, PL, NIN>(la[0], lb[0], a, 0, rl, lane, cx);
 #pragma unroll
@@ -412,6 +427,7 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
   for (int f = 0; f < NPARK; ++f)
 #pragma unroll
     for (int e = 0; e < E; ++e) q[f][e] = park[team][f][e * LANES + lane];
+  }
 
   T o[NOUT][E];
 #pragma unroll
@@ -428,7 +444,6 @@ __global__ void __launch_bounds__(64) row_phys_fast(const RowArgs<T> a, long nro
   // XF: an x-space forcing joins its product before the r2c (register
   // qq * RL + rr holds the c2r passes' final position of (lane, qq, rr)); a
   // variant of its own, so that the kernels without one keep their registers
-  constexpr int RLB = PL::R[PL::NP - 1];
 #pragma unroll
   for (int f = 0; f < NOUT; ++f) {
     if (XF && a.xadd[f]) {

@@ -100,6 +100,7 @@ SIGNATURES = {
     "ddx_step": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
     "ddx_step_noalloc": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _I, _D]),
     "ddx_step_xforced": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _PP, _I, _D]),
+    "ddx_step_xin": (_I, [_VP, _I, ctypes.POINTER(Params), _PP, _PP, _PP, _PP, _PP, _PP, _PP, _D]),
     "ddx_kcombine": (_I, [_VP, _VP, _I, _PP, _DP, ctypes.POINTER(_I), _I]),
     "ddx_xcombine": (_I, [_VP, _VP, _I, _PP, _PP, _DP, _I]),
     "ddx_xreduce": (_I, [_VP, _VP, _I, _DP]),
@@ -513,15 +514,25 @@ class Plan:
                             ptr_array([int(d) for d in deriv]), fz))
 
     def step(self, integrator, params, state, stage, deriv1, deriv2, nsteps, dt, forcing=None,
-             xforcing=None):
+             xforcing=None, xin=None):
         """ddx_step: deriv2=None leaves the second right-hand side unstored.
         xforcing (x-space arrays, one per state component or None) selects
-        ddx_step_xforced."""
+        ddx_step_xforced; xin (the x-space inputs of the first right-hand
+        side, 2D, one step) selects ddx_step_xin."""
         st = ptr_array([int(s) for s in stage]) if stage is not None else None
         d2 = ptr_array([int(d) for d in deriv2]) if deriv2 is not None else None
         fz = None
         if forcing is not None:
             fz = ptr_array([int(f) if f is not None else None for f in forcing])
+        if xin is not None:
+            if int(nsteps) != 1:
+                raise DDXError(-1, "x-space inputs apply to a single step")
+            xz = ptr_array([int(f) if f is not None else None for f in xforcing]) if xforcing is not None else None
+            _check(_lib.ddx_step_xin(self.handle, integrator, ctypes.byref(params),
+                                     ptr_array([int(s) for s in state]), st,
+                                     ptr_array([int(d) for d in deriv1]), d2,
+                                     fz, xz, ptr_array([int(x) for x in xin]), float(dt)))
+            return
         if xforcing is not None:
             xz = ptr_array([int(f) if f is not None else None for f in xforcing])
             _check(_lib.ddx_step_xforced(self.handle, integrator, ctypes.byref(params),

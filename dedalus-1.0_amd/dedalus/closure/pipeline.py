@@ -380,9 +380,10 @@ class CoarseRun:
                 unpack(out, 2, [int(b) for b in self.fx], True, scale=self.out_scale,
                        add=[int(b) for b in self.fcx])
             # the x-space sum enters the step's products, whose forward
-            # transforms carry it (no transform of its own)
+            # transforms carry it (no transform of its own); the features
+            # (u, v, omega) are the first right-hand side's x-space inputs
             C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,
-                   1, self.dt, xforcing=self.fx)
+                   1, self.dt, xforcing=self.fx, xin=self.xb)
             return
         C.step(_hip.RK2MID, self.pipe.prm, self.state, self.bufs[0], self.bufs[1], None,
                1, self.dt, forcing=forcing)

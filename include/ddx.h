@@ -310,6 +310,23 @@ int ddx_step_xforced(ddx_plan* plan, int integrator, const ddx_params* prm, void
                      void* const* stage, void* const* deriv1, void* const* deriv2,
                      const void* const* forcing, const void* const* xforcing, int nsteps,
                      double dt);
+/* One ddx_step_xforced step (2D grids) whose first right-hand side takes its
+ * x-space inputs from the caller instead of transforming the state: xin[q]
+ * (one per input of the physics' x-row products, in this order: u_x, u_y,
+ * omega = d_x u_y - d_y u_x; then T, d_x T, d_y T (Boussinesq) or B_x, B_y,
+ * J = d_x B_y - d_y B_x (MHD)) in the plan's x layout, as ddx_bwd_combine
+ * writes them from the same state.  A caller that already holds these fields
+ * (the learned closure computes them for its network) saves the backward
+ * column pass of that right-hand side.  The results equal ddx_step_xforced's
+ * to rounding (a backward transform pairs rows where the step pairs fields).
+ * Plans where the size-specialised x-row kernel does not apply (x extents
+ * other than 64, 128, 256, 512; spherical rule) ignore xin.  No counterpart
+ * in the reference (its right-hand side always transforms the state,
+ * physics.py:527-586). */
+int ddx_step_xin(ddx_plan* plan, int integrator, const ddx_params* prm, void* const* state,
+                 void* const* stage, void* const* deriv1, void* const* deriv2,
+                 const void* const* forcing, const void* const* xforcing, const void* const* xin,
+                 double dt);
 /* ddx_step that allocates nothing: no hipMalloc, hipFree or other allocating
  * call, and no internal graph capture (the kernels are launched directly on
  * the plan's stream, so a caller may capture the call in a graph of its own).

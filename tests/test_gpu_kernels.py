@@ -764,3 +764,102 @@ def test_steps_with_x_space_forcing(kind, shape, p, integ):
         with pytest.raises(hp.DDXError):
             plan.step(integ_c, prm, st, [plan.alloc_k() for _ in st], [plan.alloc_k() for _ in st],
                       None, 1, 4e-3, xforcing=bad)
+
+
+XIN_RECIPES = {
+    'hydro': [[(0, -1, 1.0)], [(1, -1, 1.0)], [(1, 0, 1.0), (0, 1, -1.0)]],
+    'boussinesq': [[(0, -1, 1.0)], [(1, -1, 1.0)], [(1, 0, 1.0), (0, 1, -1.0)],
+                   [(2, -1, 1.0)], [(2, 0, 1.0)], [(2, 1, 1.0)]],
+    'mhd': [[(0, -1, 1.0)], [(1, -1, 1.0)], [(1, 0, 1.0), (0, 1, -1.0)],
+            [(2, -1, 1.0)], [(3, -1, 1.0)], [(3, 0, 1.0), (2, 1, -1.0)]],
+}
+
+
+@pytest.mark.parametrize("kind,shape,p,integ,prec,batch", [
+    ('hydro', (64, 64), {'nu': 1e-3}, 'RK2mid', 'f64', None),
+    ('hydro', (32, 128), {'nu': 2e-3}, 'RK2trap', 'f64', None),
+    ('hydro', (64, 64), {'nu': 1e-3}, 'RK2mid', 'f64', 3),
+    ('hydro', (48, 256), {'nu': 1e-3}, 'RK2mid', 'f32', None),
+    ('boussinesq', (64, 64), {'nu': 1e-3, 'kappa': 1e-3, 'boussinesq_direction': 'y'}, 'RK2mid', 'f64', None),
+    ('mhd', (16, 128), {'nu': 1e-3, 'eta': 1e-3}, 'RK2trap', 'f64', None),
+    ('hydro', (30, 20), {'nu': 2e-3}, 'RK2mid', 'f64', None),
+], ids=str)
+def test_step_with_x_space_inputs(kind, shape, p, integ, prec, batch):
+    """ddx_step_xin, given the backward transforms of the first right-hand
+    side's inputs (ddx_bwd_combine of the state with the physics' recipes),
+    equals ddx_step_xforced to rounding (the transform pairs rows where the
+    step pairs fields); and it reads them: inputs of another state change the
+    result.  An x extent without the size-specialised x-row kernel (20)
+    ignores them, bit for bit."""
+    hp = hip()
+    nd = len(shape)
+    L = LENGTHS[nd]
+    case("x-space step inputs", kind, shape, integ, prec, batch)
+    precision = hp.F64 if prec == 'f64' else hp.F32
+    plan = hp.Plan(shape, L, precision, batch=batch) if batch else hp.Plan(shape, L, precision)
+    prm = params_struct(hp, kind, nd, p)
+    integ_c = hp.RK2MID if integ == 'RK2mid' else hp.RK2TRAP
+
+    def upload(seed):
+        kf = kfields(kind, shape, seed, L)
+        bufs = []
+        for name in FIELDS[kind]:
+            for arr in kf[name]:
+                b = plan.alloc_k()
+                plan.upload_k(b, np.stack([arr] * batch) if batch else arr)
+                bufs.append(b)
+        return bufs
+
+    nforce = nd + (1 if kind == 'boussinesq' else 0)
+    xf = []
+    for i in range(nforce):
+        xb = plan.alloc_x()
+        xs = (batch,) + tuple(shape) if batch else shape
+        plan.upload_x(xb, 0.3 * random_x(xs, 700 + i))
+        xf.append(xb)
+    ns = sum(nd if n in ('u', 'B') else 1 for n in FIELDS[kind])
+    xforcing = xf + [None] * (ns - nforce)
+    recipes = XIN_RECIPES[kind]
+    results = []
+    for mode in ('plain', 'xin', 'other'):
+        st = upload(41)
+        xin = [plan.alloc_x() for _ in recipes]
+        plan.bwd_combine(upload(43) if mode == 'other' else st, recipes, xin)
+        stage = [plan.alloc_k() for _ in st]
+        d1 = [plan.alloc_k() for _ in st]
+        if mode == 'plain':
+            plan.step(integ_c, prm, st, stage, d1, None, 1, 4e-3, xforcing=xforcing)
+        else:
+            plan.step(integ_c, prm, st, stage, d1, None, 1, 4e-3, xforcing=xforcing, xin=xin)
+        results.append([plan.download_k(b) for b in st])
+    tol = 1e-13 if prec == 'f64' else 1e-5
+    applies = shape[-1] in (64, 128, 256, 512)
+    for c, (a, b, o) in enumerate(zip(*results)):
+        e = rel_l2(b, a)
+        print("  component %d: xin vs plain %.2e, other state's xin vs plain %.2e" % (c, e, rel_l2(o, a)))
+        if applies:
+            assert e < tol, (c, e)
+        else:
+            assert np.array_equal(a, b) and np.array_equal(a, o)
+    if applies:
+        assert max(rel_l2(o, a) for a, o in zip(results[0], results[2])) > 1e-6
+    # one step only; 2D only
+    st = upload(41)
+    xin = [plan.alloc_x() for _ in recipes]
+    with pytest.raises(hp.DDXError):
+        plan.step(integ_c, prm, st, [plan.alloc_k() for _ in st], [plan.alloc_k() for _ in st], None,
+                  2, 4e-3, xforcing=xforcing, xin=xin)
+
+
+def test_step_x_space_inputs_3d_refused():
+    hp = hip()
+    case("x-space step inputs refused in 3D")
+    plan = hp.Plan((16, 16, 16), LENGTHS[3])
+    prm = params_struct(hp, 'hydro', 3, {'nu': 1e-3})
+    st = [plan.alloc_k() for _ in range(3)]
+    for b in st:
+        plan.zero(b)
+    xin = [plan.alloc_x() for _ in range(6)]
+    with pytest.raises(hp.DDXError):
+        plan.step(hp.RK2MID, prm, st, [plan.alloc_k() for _ in st], [plan.alloc_k() for _ in st], None,
+                  1, 1e-3, xin=xin)
