@@ -98,6 +98,7 @@ class Ensemble:
         self.dt = float(dt)
         self.length = tuple(length)
         self.graph = None
+        self.graphs = []
 
     @property
     def points(self):
@@ -119,8 +120,11 @@ class Ensemble:
             for g in self.groups:
                 g.step(self.params, 1, self.dt)
 
-    def capture(self, concurrent=True):
-        """One graph holding one RK2mid step of every group.
+    def capture(self, concurrent=True, steps_per_graph=1):
+        """Graphs holding RK2mid steps of every group: one of one step, and
+        (steps_per_graph > 1) one of that many consecutive steps, which
+        advance() replays for the bulk of a run, so that a run of
+        steps_per_graph steps is one graph launch.
 
         The capture first takes one eager step (it allocates the workspaces,
         which cannot be allocated inside a graph), so every member advances
@@ -129,39 +133,50 @@ class Ensemble:
 
         concurrent: the groups are independent, so each group other than the
         first is captured on a side stream forked from and joined back to the
-        ensemble's stream; the graph then lets the small groups' launches,
-        which do not fill the GPU, run beside the large group's."""
+        ensemble's stream (once per step); the graph then lets the small
+        groups' launches, which do not fill the GPU, run beside the large
+        group's."""
         import torch
         for g in self.groups:
             g.plan.set_graph(False)
         side = [torch.cuda.Stream() for _ in self.groups[1:]] if concurrent else []
+        k = max(1, int(steps_per_graph))
+
+        def one_step():
+            if not side:
+                self.step_eager(1)
+                return
+            fork = torch.cuda.Event()
+            fork.record(self.stream)
+            joins = []
+            for g, s in zip(self.groups[1:], side):
+                s.wait_event(fork)
+                g.plan.set_stream(s.cuda_stream)
+                g.step(self.params, 1, self.dt)
+                e = torch.cuda.Event()
+                e.record(s)
+                joins.append(e)
+            self.groups[0].step(self.params, 1, self.dt)
+            for e in joins:
+                self.stream.wait_event(e)
+
+        graphs = []
         with torch.cuda.stream(self.stream):
             self.step_eager(1)                       # workspaces, twiddles
             self.stream.synchronize()
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph, stream=self.stream):
-                if not side:
-                    self.step_eager(1)
-                else:
-                    fork = torch.cuda.Event()
-                    fork.record(self.stream)
-                    joins = []
-                    for g, s in zip(self.groups[1:], side):
-                        s.wait_event(fork)
-                        g.plan.set_stream(s.cuda_stream)
-                        g.step(self.params, 1, self.dt)
-                        e = torch.cuda.Event()
-                        e.record(s)
-                        joins.append(e)
-                    self.groups[0].step(self.params, 1, self.dt)
-                    for e in joins:
-                        self.stream.wait_event(e)
+            for n in sorted({1, k}):
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph, stream=self.stream):
+                    for _ in range(n):
+                        one_step()
+                graphs.append((n, graph))
         for g in self.groups[1:]:
             g.plan.set_stream(self.stream.cuda_stream)
         self.stream.synchronize()
-        self.graph = graph
-        self._side = side                            # kept alive with the graph
-        return graph
+        self.graphs = sorted(graphs, key=lambda t: -t[0])
+        self.graph = self.graphs[-1][1]
+        self._side = side                            # kept alive with the graphs
+        return self.graph
 
     def autotune(self, steps=4, reps=2):
         """Choose, per group, the kernel variants that run its step fastest:
@@ -176,6 +191,8 @@ class Ensemble:
         import itertools
         import torch
         self.graph = None
+        self.graphs = []
+        self.graphs = []
         chosen = {}
         with torch.cuda.stream(self.stream):
             for g in self.groups:
@@ -214,13 +231,18 @@ class Ensemble:
         return chosen
 
     def advance(self, nsteps):
+        """nsteps RK2mid steps of every member: the captured graphs (the
+        longest first), or eager launches before capture()."""
         import torch
+        n = int(nsteps)
         with torch.cuda.stream(self.stream):
-            for _ in range(int(nsteps)):
-                if self.graph is not None:
-                    self.graph.replay()
-                else:
-                    self.step_eager(1)
+            if self.graph is None:
+                self.step_eager(n)
+                return
+            for k, g in self.graphs:
+                while n >= k:
+                    g.replay()
+                    n -= k
 
     def sync(self):
         self.stream.synchronize()

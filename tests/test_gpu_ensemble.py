@@ -127,19 +127,21 @@ def test_members_match_oracle():
 
 
 def test_ensemble_graph_matches_eager_and_oracle():
-    """Two groups (32^2 x 3, 64^2 x 2) in FP32: the one-graph step equals eager
-    group steps bit for bit across replays; members stay within FP32 error of
-    the FP64 oracle."""
+    """Two groups (32^2 x 3, 64^2 x 2) in FP32: replays of one-step and
+    multi-step graphs equal eager group steps bit for bit; members stay within
+    FP32 error of the FP64 oracle."""
     hip()
     from dedalus.ensemble import Ensemble, vorticity_member
     case("ensemble graph vs eager vs oracle")
     groups = (((32, 32), 3), ((64, 64), 2))
     runs = []
-    for graph in (False, True):
+    # eager; one-step graphs; three-step graphs (4 steps = one three-step
+    # replay and one one-step replay); a four-step graph (one replay per call)
+    for graph in (None, 1, 3, 4):
         ens = Ensemble(groups, nu=5e-4, dt=1e-3, precision="f32")
         ens.seed_members(1000)
         if graph:
-            ens.capture()
+            ens.capture(steps_per_graph=graph)
             ens.seed_members(1000)
         snaps = []
         for call in range(3):
@@ -147,10 +149,11 @@ def test_ensemble_graph_matches_eager_and_oracle():
             ens.sync()
             snaps.append([g.read() for g in ens.groups])
         runs.append(snaps)
-    for a, b in zip(*runs):
-        for ga, gb in zip(a, b):
-            for (ua, va), (ub, vb) in zip(ga, gb):
-                assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            for ga, gb in zip(a, b):
+                for (ua, va), (ub, vb) in zip(ga, gb):
+                    assert np.array_equal(ua, ub) and np.array_equal(va, vb)
     # FP32 members vs the FP64 oracle after 12 steps
     m = 0
     for (shape, n), gsnap in zip(groups, runs[1][-1]):
