@@ -137,16 +137,17 @@ def test_golden_fp32_hydro3d_128():
 
 
 def test_golden_fp32_ensemble768():
-    """The 768-member FP32 ensemble as benchmarked (three batched groups, one
-    graph replay per step) for its 500 steps; members 0, 1, 256 and 512
-    against their FP64 oracle trajectories."""
+    """The 768-member FP32 ensemble as benchmarked (three batched groups
+    captured together, many steps per graph launch: here 100, so the run is
+    five replays) for its 500 steps; members 0, 1, 256 and 512 against their
+    FP64 oracle trajectories."""
     _import_maker()
     from dedalus.ensemble import ENSEMBLE_768, ensemble_768
     import make_fixtures
     hip()
     ens = ensemble_768()
     print("case: golden FP32 ensemble_768, 500 steps", flush=True)
-    ens.capture()                       # takes one eager step: reseed after it
+    ens.capture(steps_per_graph=100)    # takes one eager step: reseed after it
     ens.seed_members(ENSEMBLE_768["seed0"])
     ens.advance(ENSEMBLE_768["nsteps"])
     ens.sync()
